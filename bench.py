@@ -1,0 +1,316 @@
+#!/usr/bin/env python3
+"""QPS at recall@10 >= 0.95 on one MI355X, beside the reference's CPU search.
+
+Workload (default): 1,000,000 x 128 uint8 vectors from the 64-cluster Gaussian
+mixture quantised to [0,255] (seed 1), L2, M=32, efConstruction=256, 10,000
+queries, k=10 -- a graph built on the CPU by the reference and searched on
+the GPU.
+
+Steps
+  1. generate the vectors (pickle_amd.datasets, seeded);
+  2. CPU baseline leg: the reference's own HNSWGraph::Build (oracle/_ref, the
+     reference headers compiled from source) on all host threads, timed, and
+     written in the reference serializer layout;
+  3. exhaustive ground truth on the GPU (pk_brute_force, exact int32);
+  4. load the graph into HBM, sweep ef, pick the smallest ef whose recall@10
+     (label matches, benchmark/hnsw_pickle.cpp:94-112) reaches the target;
+  5. time --steps launches of the whole query batch at that ef with HIP events
+     on the index stream (inputs and outputs resident in HBM);
+  6. CPU baseline leg: the reference's AnnSearch over the same queries, graph,
+     ef and k on the host cores.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QPS at recall@10>=0.95 (1 GPU vs host CPU); p99 latency; achieved HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "gm1m_u8": dict(desc="1M x 128 uint8, 64-cluster gaussian mixture quantised to [0,255] (seed 1), L2, "
+                         "M=32, efC=256, 10k queries, k=10, CPU-built graph", n=1_000_000, nq=10_000),
+    "gm100k_f32": dict(desc="100k x 128 float32 (rounded), 64-cluster gaussian mixture (seed 0), L2, M=16, "
+                            "efC=200, 1k queries, k=10", n=100_000, nq=1_000),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    return min(os.cpu_count() or 1, 16)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def has_avx512() -> bool:
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return all(f in flags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl", "avx512cd"))
+
+
+# --------------------------------------------------------------------------
+# cpu_baseline leg: the reference compiled from its sources (oracle/_ref)
+# --------------------------------------------------------------------------
+
+def ref_binary():
+    from oracle import oracle as O
+
+    if has_avx512() and os.path.exists(O.REF_BIN):
+        return O.REF_BIN, "avx512"
+    if os.path.exists(O.REF_BIN_GENERIC):
+        return O.REF_BIN_GENERIC, "generic"
+    return None, None
+
+
+def cpu_build(binary, base_path, n, dim, dtype, metric, M, efc, threads, out):
+    cmd = [binary, "build", "--base", base_path, "--n", str(n), "--dim", str(dim), "--dtype", dtype,
+           "--metric", metric, "--M", str(M), "--efc", str(efc), "--threads", str(threads), "--out", out,
+           "--shuffle", "1"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference build failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_search(binary, graph, base_path, q_path, n, nq, dim, dtype, k, ef, threads, repeat, out):
+    cmd = [binary, "search", "--graph", graph, "--base", base_path, "--queries", q_path, "--n", str(n),
+           "--nq", str(nq), "--dim", str(dim), "--dtype", dtype, "--k", str(k), "--ef", str(ef),
+           "--threads", str(threads), "--repeat", str(repeat), "--out", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference search failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout)[0]
+
+
+def read_ref_results(path, nq, k):
+    raw = np.fromfile(path, dtype=np.uint8).reshape(nq, 4 + 8 * k)
+    pairs = raw[:, 4:].copy().reshape(nq, k, 8)
+    return pairs[:, :, :4].copy().view(np.int32)[:, :, 0]
+
+
+# --------------------------------------------------------------------------
+
+def recall_at_k(ids, gt, k):
+    """benchmark/hnsw_pickle.cpp:94-112: a returned id counts if it is one of
+    the first k ground-truth labels (ties are not credited)."""
+    hit = 0
+    for i in range(ids.shape[0]):
+        hit += len(set(ids[i, :k].tolist()) & set(gt[i, :k].tolist()))
+    return hit / (k * ids.shape[0])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="gm1m_u8", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=None, help="override the base size (smaller runs)")
+    ap.add_argument("--nq", type=int, default=None)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--recall-target", type=float, default=0.95)
+    ap.add_argument("--efs", default="10,12,14,16,20,24,28,32,40,48,64,96,128,192,256,384,512")
+    ap.add_argument("--ef", type=int, default=0, help="fixed ef (skips the sweep)")
+    ap.add_argument("--graph-cache", default=None, help="directory to reuse / keep the CPU-built graph")
+    ap.add_argument("--no-cpu-search", action="store_true", help="skip the CPU search baseline")
+    ap.add_argument("--cpu-repeat", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (run with --gpus 1). "
+              "Multi-GPU sharding is described in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    from pickle_amd import abi, datasets
+
+    wl = WORKLOADS[args.workload]
+    n = args.n or wl["n"]
+    nq = args.nq or wl["nq"]
+    k = args.k
+    t0 = time.time()
+    base, queries, params = datasets.workload(args.workload, n=n, nq=nq)
+    dtype_name = {np.dtype(np.uint8): "u8", np.dtype(np.int8): "i8", np.dtype(np.float32): "f32",
+                  np.dtype(np.float16): "f16"}[base.dtype]
+    dim = base.shape[1]
+    metric = params["metric"]
+    M, efc = params["M"], params["efc"]
+    log(f"workload {args.workload}: n={n} nq={nq} dim={dim} {dtype_name} {metric} M={M} efC={efc} "
+        f"(generated in {time.time() - t0:.1f}s)")
+
+    threads = host_threads()
+    binary, isa = ref_binary()
+    if binary is None:
+        sys.exit("the reference build (oracle/_ref) is missing: run __graft_entry__.build() where the "
+                 "reference sources are")
+
+    work = args.graph_cache or tempfile.mkdtemp(prefix="pk_bench_")
+    os.makedirs(work, exist_ok=True)
+    tag = f"{args.workload}_n{n}"
+    base_path = os.path.join(work, f"{tag}.base")
+    q_path = os.path.join(work, f"{tag}_q{nq}.queries")
+    graph_path = os.path.join(work, f"{tag}.graph")
+    meta_path = graph_path + ".build.json"
+    base.tofile(base_path)
+    queries.tofile(q_path)
+    if os.path.exists(graph_path) and os.path.exists(meta_path):
+        build_info = json.load(open(meta_path))
+        log(f"reusing graph {graph_path} (built in {build_info['build_seconds']:.1f}s)")
+    else:
+        log(f"reference CPU build ({isa}, {threads} threads) ...")
+        build_info = cpu_build(binary, base_path, n, dim, dtype_name, metric, M, efc, threads, graph_path)
+        json.dump(build_info, open(meta_path, "w"))
+        log(f"built in {build_info['build_seconds']:.1f}s, {build_info['layers']} layers")
+
+    mcode = abi.L2 if metric == "l2" else abi.IP
+    idx = abi.Index(base.dtype, mcode, dim, M, efc, n)
+    idx.set_base(base)
+    idx.load_file(graph_path)
+    info = idx.info()
+    log(f"graph in HBM: layers {info['sizes']}")
+
+    t1 = time.time()
+    gt_ids, gt_d = idx.brute_force(queries)
+    if base.dtype.kind == "f":  # FP64 re-rank of the FP32 candidates
+        A, Q = base.astype(np.float64), queries.astype(np.float64)
+        for i in range(nq):
+            c = gt_ids[i]
+            d64 = ((A[c] - Q[i]) ** 2).sum(1) if metric == "l2" else -(A[c] @ Q[i])
+            gt_ids[i] = c[np.lexsort((c, d64))]
+    log(f"ground truth (GPU brute force) in {time.time() - t1:.1f}s")
+
+    # HBM-resident buffers for the timed runs
+    d_q = abi.DeviceBuffer.from_array(queries)
+    d_ids = abi.DeviceBuffer(nq * k * 4)
+    d_dist = abi.DeviceBuffer(nq * k * 4)
+    d_cnt = abi.DeviceBuffer(nq * 4)
+    d_ctr = abi.DeviceBuffer(nq * abi.COUNTER_DTYPE.itemsize)
+
+    def run_once(ef, counters=False):
+        idx.search_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt, d_ctr if counters else None)
+
+    sweep = []
+    efs = [args.ef] if args.ef else [int(e) for e in args.efs.split(",")]
+    chosen = None
+    for ef in efs:
+        run_once(ef)
+        idx.sync()
+        idx.timer_begin()
+        run_once(ef)
+        ms = idx.timer_end()
+        ids = d_ids.to_array(np.int32, (nq, k))
+        r = recall_at_k(ids, gt_ids, k)
+        sweep.append([ef, round(r, 4), round(nq / (ms / 1e3), 1)])
+        log(f"ef={ef:4d} recall@{k}={r:.4f} qps={nq / (ms / 1e3):,.0f}")
+        if r >= args.recall_target and chosen is None:
+            chosen = ef
+            if not args.ef:
+                break
+    if chosen is None:
+        chosen = efs[-1]
+        log(f"no ef reached recall {args.recall_target}; using ef={chosen}")
+    recall = [s[1] for s in sweep if s[0] == chosen][0]
+    if idx.device_error():
+        sys.exit("device reported a search failure")
+
+    # work counters (untimed pass, as the reference's second profiler pass)
+    run_once(chosen, counters=True)
+    idx.sync()
+    ctr = d_ctr.to_array(abi.COUNTER_DTYPE, (nq,))
+    elem = base.dtype.itemsize
+    hops = ctr["hops"].sum(axis=1).astype(np.int64)
+    nbr = ctr["neighbors"].sum(axis=1).astype(np.int64)
+    dists = ctr["dists"].sum(axis=1).astype(np.int64)
+    # algorithmic bytes (reference Profiler units): 4 B per adjacency entry
+    # scanned, 8 B of node metadata per hop, one row per distance, the query
+    alg_bytes = int((4 * nbr + 8 * hops + elem * dim * dists).sum() + nq * elem * dim)
+    khz = abi.wall_clock_khz()
+    lat_ms = ctr["latency_ticks"].astype(np.float64) / khz
+
+    # timed region
+    for _ in range(args.warmup):
+        run_once(chosen)
+    idx.sync()
+    tw0 = time.perf_counter()
+    idx.timer_begin()
+    for _ in range(args.steps):
+        run_once(chosen)
+    ev_ms = idx.timer_end()
+    idx.sync()
+    wall = time.perf_counter() - tw0
+    qps = nq * args.steps / wall
+    kernel_s = ev_ms / 1e3 / args.steps
+    achieved = alg_bytes / kernel_s / 1e9
+    grid, lds = idx.last_launch()
+    log(f"timed: {args.steps} x {nq} queries at ef={chosen}: {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
+        f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
+
+    # cpu_baseline leg: the reference's AnnSearch on the host cores
+    cpu = None
+    if not args.no_cpu_search:
+        out = os.path.join(work, "cpu_res")
+        st = cpu_search(binary, graph_path, base_path, q_path, n, nq, dim, dtype_name, k, chosen, threads,
+                        args.cpu_repeat, out)
+        cpu_ids = read_ref_results(f"{out}.ef{chosen}", nq, k)
+        cpu_recall = recall_at_k(cpu_ids, gt_ids, k)
+        same = float(np.mean(np.all(cpu_ids == d_ids.to_array(np.int32, (nq, k)), axis=1)))
+        cpu = {"value": round(st["qps"], 1), "unit": "queries/s", "cores": threads, "kind": "reference",
+               "sample": f"all {nq} queries, ef={chosen}, k={k}, best of {args.cpu_repeat} runs, reference "
+                         f"AnnSearch compiled from source ({isa} distance path) on {cpu_model()}",
+               "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
+               "ids_identical_to_gpu": round(same, 4), "build_seconds": round(build_info["build_seconds"], 2),
+               "build_threads": build_info["threads"]}
+        log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads, recall {cpu_recall:.4f}, "
+            f"GPU ids identical on {same * 100:.2f}% of queries")
+
+    line = {
+        "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype_name, "data": "synthetic",
+        "config": {"workload": wl["desc"] if (n, nq) == (wl["n"], wl["nq"]) else f"{wl['desc']} (n={n}, nq={nq})",
+                   "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc, "k": k, "ef": chosen,
+                   "recall@10": recall, "search": "exact (reference heap order)",
+                   "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
+                   "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
+                   "batch_latency_ms": round(wall / args.steps * 1e3, 4),
+                   "sweep_ef_recall_qps": sweep},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_search", "kernel_ms": round(kernel_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),
+                                   "dists": round(float(dists.mean()), 2)}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

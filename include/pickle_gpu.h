@@ -1,0 +1,190 @@
+/*
+ * pickle_gpu.h -- C-ABI of the MI355X HNSW query and insertion path.
+ *
+ * libpickle_amd.so implements pickle v2's HNSWGraph search and insertion
+ * (reference include/hnsw.hpp, include/hnsw_layer.hpp) with HIP kernels for
+ * gfx950.  Every entry point takes plain pointers and sizes, returns a status
+ * (PK_OK = 0, negative on error, never aborts) and leaves the message for
+ * pk_last_error().  The library owns the HBM-resident index and its scratch
+ * memory; callers own every buffer they pass in.  Calls on one handle are
+ * serialised on the handle's HIP stream.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the reference repository).
+ */
+#ifndef PICKLE_GPU_H
+#define PICKLE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PK_OK 0
+#define PK_ERR_ARG -1      /* bad argument (shape, dtype, metric, range) */
+#define PK_ERR_HIP -2      /* a HIP runtime call failed */
+#define PK_ERR_NOMEM -3    /* device allocation failed */
+#define PK_ERR_STATE -4    /* call not valid in the handle's current state */
+#define PK_ERR_IO -5       /* file read / write failed or malformed file */
+#define PK_ERR_CAPACITY -6 /* a fixed capacity would be exceeded */
+#define PK_ERR_DEVICE -7   /* a kernel reported a failure (see pk_last_error) */
+
+/* element types: DataType, include/common.hpp:21-26 (same codes) */
+#define PK_UINT8 0
+#define PK_INT8 1
+#define PK_FLOAT32 2
+#define PK_FLOAT16 3
+
+/* metrics: DistFunc, include/common.hpp:28-33 (same codes; L1 unsupported,
+ * as in include/distance/avx512.hpp:233-243) */
+#define PK_L2 1
+#define PK_IP 2
+
+#define PK_MAX_LEVELS 16
+
+typedef struct pk_index pk_index;
+
+/* Per-query work counters with the reference Profiler's definitions
+ * (include/profiler.hpp:13-69, updated at include/hnsw.hpp:438-442 and
+ * :488-492): hops = expanded nodes, neighbors = adjacency entries scanned,
+ * dists = distance evaluations.  Index [0] is layer 0, [1] all upper layers.
+ * latency_ticks: device wall-clock ticks (pk_wall_clock_khz) from the moment
+ * a wave takes the query to the moment its results are written. */
+typedef struct {
+    int32_t hops[2];
+    int32_t neighbors[2];
+    int32_t dists[2];
+    int32_t latency_ticks;
+} pk_query_counters;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *pk_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int pk_version(void);
+
+/* HNSWGraph(max_degree, build_ef, df) + InitBuffer(max_nodes)
+ * (include/hnsw.hpp:27-38, 222-245).  M is the reference's max_degree
+ * argument (layer-0 rows hold 2M ids, upper rows M); efc is build_ef;
+ * capacity is the number of external ids the index can ever hold. */
+int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capacity,
+                    pk_index **out);
+
+/* Frees the index and all device memory it owns. */
+int pk_index_destroy(pk_index *h);
+
+/* Base vectors by external id: the NDArray the reference keeps as m_data
+ * (include/hnsw.hpp:140, 497-501).  Host rows, n x dim of the index dtype.
+ * Rows [first, first + n) are (over)written. */
+int pk_index_set_base(pk_index *h, const void *host_rows, int64_t first, int64_t n);
+
+/* One layer of a graph in the serializer's layout (include/serializer.hpp:
+ * 27-55) plus v2's per-node heuristic count and vector flag
+ * (include/hnsw_layer.hpp:60-65, 83).  adj/dist hold adj_len >= n * D items:
+ * rows of stride D, plus any slots a degree above D reads past the last row
+ * (include/hnsw_layer.hpp:441-447 lets a row grow past D).  has_vector[i]
+ * != 0 means the layer's copy of node i is base row ext[i]; 0 means zeros.
+ * dist and heur may be NULL for a search-only index. */
+int pk_index_load_layer(pk_index *h, int level, int64_t n, int D, int64_t adj_len,
+                        const int32_t *ext, const int32_t *degree, const int16_t *heur,
+                        const int32_t *adj, const float *dist, const uint8_t *has_vector);
+
+/* m_enter_level (include/hnsw.hpp:133, 142-151).  Loads are complete after
+ * this call: the search kernels read the layers from here on. */
+int pk_index_set_enter_level(pk_index *h, int enter_level);
+
+/* Reads a graph file in the serializer layout plus its ".v2meta" sidecar
+ * (the format is described in pickle_amd/serializer.py); base rows must
+ * already be set.  Replaces Serializer::from_disk (include/serializer.hpp:57-81). */
+int pk_index_load_file(pk_index *h, const char *path);
+
+/* Writes the index's graph in the same format.  Replaces
+ * Serializer::to_disk (include/serializer.hpp:27-55). */
+int pk_index_save_file(pk_index *h, const char *path);
+
+/* Graph shape. */
+int pk_index_info(pk_index *h, int *nlayers, int *enter_level, int64_t *layer_sizes,
+                  int *layer_degrees);
+
+/* Copies layer `level` back to host arrays (any pointer may be NULL).
+ * adj/dist receive n * D items. */
+int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, int16_t *heur,
+                        int32_t *adj, float *dist);
+
+/* Batched AnnSearch (include/hnsw.hpp:509-541 and SelectSimpleExt :118-125):
+ * for each query i, greedy descent through the upper layers, the ef[i]-bounded
+ * layer-0 beam search and the ascending top-k[i] with external ids.  Results
+ * are bit-identical to the reference for u8/i8 and for f16/f32 whenever the
+ * distance order reproduces the reference's (dim % 16 < 8).
+ *   queries   nq x dim rows of the index dtype
+ *   k, ef     per-query arrays (or NULL with k_all / ef_all used for all)
+ *   out_ids   nq x k_stride external ids, -1 past the result count
+ *   out_dists nq x k_stride distances (float32; -IP for IP), +inf past it
+ *   out_count nq result counts (min(k, |W|)), may be NULL
+ *   counters  nq per-query counters, may be NULL
+ * Host pointers; the call copies in, runs and copies out synchronously. */
+int pk_search(pk_index *h, const void *queries, int64_t nq, const int32_t *k, const int32_t *ef,
+              int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t *out_ids, float *out_dists,
+              int32_t *out_count, pk_query_counters *counters);
+
+/* Same as pk_search with every pointer a device pointer (HBM-resident
+ * inputs and outputs); enqueued on the handle's stream, returns without
+ * waiting.  k/ef may be NULL (k_all/ef_all then apply). */
+int pk_search_device(pk_index *h, const void *d_queries, int64_t nq, const int32_t *d_k,
+                     const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t k_stride,
+                     int32_t *d_out_ids, float *d_out_dists, int32_t *d_out_count,
+                     pk_query_counters *d_counters);
+
+/* pk_search_device with per-query ef on the device: ef_max bounds every
+ * d_ef[i] (it sizes the LDS result heap).  Same reference interface. */
+int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const int32_t *d_k,
+                        const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max,
+                        int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
+                        int32_t *d_out_count, pk_query_counters *d_counters);
+
+/* Device failure flags of the last search (0 = none; 1 = a scratch capacity
+ * was exceeded, 2 = a query had k or ef out of range).  Waits for the stream. */
+int pk_device_error(pk_index *h, int *flags);
+
+/* Search launch shape: LDS candidate-heap entries, log2 of LDS visited slots
+ * and a cap on resident workgroups per CU (0 = automatic).  No reference
+ * counterpart (the reference has no device launch); set by the autotuner. */
+int pk_set_tuning(pk_index *h, int ccap, int vbits, int blocks_per_cu);
+
+/* Grid size and LDS bytes per workgroup of the last search launch. */
+int pk_last_launch(pk_index *h, int *grid, int *lds_bytes);
+
+/* Exhaustive nearest neighbours of nq host queries over the base rows set
+ * with pk_index_set_base (rows [0, highest row set)): the 32 closest per
+ * query, ascending by (distance, row id), to out_ids / out_dists (nq x 32).
+ * u8 / i8 distances are exact; f16 / f32 are FP32 sums, to be re-ranked in
+ * FP64 by the caller.  No reference counterpart on the hot path: the
+ * reference reads precomputed ground-truth files
+ * (benchmark/bench_util.hpp:196-202); this produces them. */
+int pk_brute_force(pk_index *h, const void *queries, int64_t nq, int32_t *out_ids,
+                   float *out_dists);
+
+/* Frequency of the device wall clock behind latency_ticks, in kHz. */
+int pk_wall_clock_khz(int *khz);
+
+/* Waits for all work enqueued on the handle's stream. */
+int pk_sync(pk_index *h);
+
+/* Device memory helpers for callers without their own allocator. */
+int pk_device_alloc(void **d_ptr, size_t bytes);
+int pk_device_free(void *d_ptr);
+int pk_copy_to_device(void *d_dst, const void *h_src, size_t bytes);
+int pk_copy_to_host(void *h_dst, const void *d_src, size_t bytes);
+
+/* HIP-event timing of the work enqueued between begin and end on the
+ * handle's stream; pk_timer_end waits and returns milliseconds. */
+int pk_timer_begin(pk_index *h);
+int pk_timer_end(pk_index *h, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,283 @@
+"""ctypes binding of the C-ABI in include/pickle_gpu.h (libpickle_amd.so).
+
+There is no CPU fallback: if the library is missing or a call fails, an
+exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpickle_amd.so")
+
+UINT8, INT8, FLOAT32, FLOAT16 = 0, 1, 2, 3
+L2, IP = 1, 2
+MAX_LEVELS = 16
+
+_DTYPES = {UINT8: np.uint8, INT8: np.int8, FLOAT32: np.float32, FLOAT16: np.float16}
+
+STATUS = {0: "PK_OK", -1: "PK_ERR_ARG", -2: "PK_ERR_HIP", -3: "PK_ERR_NOMEM", -4: "PK_ERR_STATE",
+          -5: "PK_ERR_IO", -6: "PK_ERR_CAPACITY", -7: "PK_ERR_DEVICE"}
+
+# every entry point the header declares (checked by the tests)
+EXPORTS = [
+    "pk_last_error", "pk_version", "pk_index_create", "pk_index_destroy", "pk_index_set_base",
+    "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
+    "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
+    "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
+    "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz",
+]
+BRUTE_K = 32
+
+
+class PickleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+COUNTER_DTYPE = np.dtype([("hops", np.int32, 2), ("neighbors", np.int32, 2), ("dists", np.int32, 2),
+                          ("latency_ticks", np.int32)])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() or python -m pickle_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t
+        sig = {
+            "pk_last_error": (C.c_char_p, []),
+            "pk_version": (C.c_int, []),
+            "pk_index_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i64, C.POINTER(vp)]),
+            "pk_index_destroy": (C.c_int, [vp]),
+            "pk_index_set_base": (C.c_int, [vp, vp, i64, i64]),
+            "pk_index_load_layer": (C.c_int, [vp, C.c_int, i64, C.c_int, i64, vp, vp, vp, vp, vp, vp]),
+            "pk_index_set_enter_level": (C.c_int, [vp, C.c_int]),
+            "pk_index_load_file": (C.c_int, [vp, C.c_char_p]),
+            "pk_index_save_file": (C.c_int, [vp, C.c_char_p]),
+            "pk_index_info": (C.c_int, [vp, vp, vp, vp, vp]),
+            "pk_index_read_layer": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
+            "pk_search": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_device_ex": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_device_error": (C.c_int, [vp, vp]),
+            "pk_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+            "pk_last_launch": (C.c_int, [vp, vp, vp]),
+            "pk_sync": (C.c_int, [vp]),
+            "pk_device_alloc": (C.c_int, [C.POINTER(vp), sz]),
+            "pk_device_free": (C.c_int, [vp]),
+            "pk_copy_to_device": (C.c_int, [vp, vp, sz]),
+            "pk_copy_to_host": (C.c_int, [vp, vp, sz]),
+            "pk_timer_begin": (C.c_int, [vp]),
+            "pk_timer_end": (C.c_int, [vp, C.POINTER(C.c_float)]),
+            "pk_brute_force": (C.c_int, [vp, vp, i64, vp, vp]),
+            "pk_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise PickleError(rc, lib().pk_last_error().decode())
+
+
+def wall_clock_khz() -> int:
+    k = C.c_int()
+    check(lib().pk_wall_clock_khz(C.byref(k)))
+    return k.value
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    for code, npd in _DTYPES.items():
+        if dt == np.dtype(npd):
+            return code
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+class DeviceBuffer:
+    """Device memory owned by Python (for HBM-resident search inputs)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().pk_device_alloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib().pk_copy_to_device(b.ptr, _p(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        check(lib().pk_copy_to_host(_p(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().pk_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Index:
+    """An HBM-resident HNSW index (one pk_index handle)."""
+
+    def __init__(self, dtype, metric: int, dim: int, M: int, efc: int, capacity: int):
+        self.dtype = np.dtype(dtype)
+        self.metric = metric
+        self.dim = dim
+        self.M = M
+        self.efc = efc
+        self.capacity = capacity
+        h = C.c_void_p()
+        check(lib().pk_index_create(dtype_code(self.dtype), metric, dim, M, efc, capacity, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().pk_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_base(self, rows: np.ndarray, first: int = 0) -> None:
+        rows = np.ascontiguousarray(rows, dtype=self.dtype)
+        check(lib().pk_index_set_base(self.h, _p(rows), first, rows.shape[0]))
+
+    def load_layer(self, level: int, ext, degree, adj, has_vector, heur=None, dist=None, D: int | None = None) -> None:
+        """adj is [rows, D] (rows >= n; rows past n are slots a degree above D
+        reads past the last row) or flat with D given."""
+        ext = np.ascontiguousarray(ext, np.int32)
+        degree = np.ascontiguousarray(degree, np.int32)
+        adj = np.ascontiguousarray(adj, np.int32)
+        n = ext.shape[0]
+        if D is None:
+            if adj.ndim != 2:
+                raise ValueError("flat adjacency needs D")
+            D = adj.shape[1]
+        flat = adj.reshape(-1)
+        has = np.ascontiguousarray(has_vector, np.uint8)
+        heur = None if heur is None else np.ascontiguousarray(heur, np.int16)
+        if dist is not None:
+            dist = np.ascontiguousarray(np.asarray(dist, np.float32).reshape(-1)[: n * D])
+        check(lib().pk_index_load_layer(self.h, level, n, D, flat.shape[0], _p(ext), _p(degree), _p(heur), _p(flat),
+                                        _p(dist), _p(has)))
+
+    def set_enter_level(self, level: int) -> None:
+        check(lib().pk_index_set_enter_level(self.h, level))
+
+    def load_file(self, path: str) -> None:
+        check(lib().pk_index_load_file(self.h, path.encode()))
+
+    def save_file(self, path: str) -> None:
+        check(lib().pk_index_save_file(self.h, path.encode()))
+
+    def info(self):
+        nl, el = C.c_int(), C.c_int()
+        sizes = np.zeros(MAX_LEVELS, np.int64)
+        degs = np.zeros(MAX_LEVELS, np.int32)
+        check(lib().pk_index_info(self.h, C.byref(nl), C.byref(el), _p(sizes), _p(degs)))
+        return {"nlayers": nl.value, "enter_level": el.value, "sizes": sizes[: nl.value].tolist(),
+                "degrees": degs[: nl.value].tolist()}
+
+    def read_layer(self, level: int) -> dict:
+        inf = self.info()
+        n, D = inf["sizes"][level], inf["degrees"][level]
+        out = {"ext": np.zeros(n, np.int32), "deg": np.zeros(n, np.int32), "heur": np.zeros(n, np.int16),
+               "adj": np.zeros((n, D), np.int32), "dist": np.zeros((n, D), np.float32)}
+        check(lib().pk_index_read_layer(self.h, level, _p(out["ext"]), _p(out["deg"]), _p(out["heur"]),
+                                        _p(out["adj"]), _p(out["dist"])))
+        return out
+
+    def set_tuning(self, ccap: int = 0, vbits: int = 0, blocks_per_cu: int = 0) -> None:
+        check(lib().pk_set_tuning(self.h, ccap, vbits, blocks_per_cu))
+
+    def last_launch(self):
+        g, l = C.c_int(), C.c_int()
+        check(lib().pk_last_launch(self.h, C.byref(g), C.byref(l)))
+        return g.value, l.value
+
+    def search(self, queries: np.ndarray, k, ef, k_stride: int | None = None, counters: bool = False):
+        """Batched AnnSearch; k and ef are ints or per-query arrays.
+
+        Returns (ids [nq, k_stride], dists [nq, k_stride], counts [nq]) and, with
+        counters=True, a structured array of per-query work counters."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq = q.shape[0]
+        k_arr = None if np.isscalar(k) else np.ascontiguousarray(k, np.int32)
+        ef_arr = None if np.isscalar(ef) else np.ascontiguousarray(ef, np.int32)
+        if k_stride is None:
+            k_stride = int(k) if k_arr is None else int(k_arr.max())
+        ids = np.empty((nq, k_stride), np.int32)
+        ds = np.empty((nq, k_stride), np.float32)
+        cnt = np.empty(nq, np.int32)
+        ctr = np.zeros(nq, COUNTER_DTYPE) if counters else None
+        check(lib().pk_search(self.h, _p(q), nq, _p(k_arr), _p(ef_arr), 0 if k_arr is not None else int(k),
+                              0 if ef_arr is not None else int(ef), k_stride, _p(ids), _p(ds), _p(cnt), _p(ctr)))
+        if counters:
+            return ids, ds, cnt, ctr
+        return ids, ds, cnt
+
+    def search_device(self, d_queries: DeviceBuffer, nq: int, k: int, ef: int, k_stride: int, d_ids: DeviceBuffer,
+                      d_dists: DeviceBuffer, d_count: DeviceBuffer | None = None, d_ctr: DeviceBuffer | None = None,
+                      d_k: DeviceBuffer | None = None, d_ef: DeviceBuffer | None = None, ef_max: int | None = None):
+        """Enqueue a search whose buffers are all device-resident (no wait)."""
+        check(lib().pk_search_device_ex(self.h, d_queries.ptr, nq, d_k.ptr if d_k else None, d_ef.ptr if d_ef else None,
+                                        k, ef, ef_max if ef_max else ef, k_stride, d_ids.ptr, d_dists.ptr,
+                                        d_count.ptr if d_count else None, d_ctr.ptr if d_ctr else None))
+
+    def brute_force(self, queries: np.ndarray):
+        """The BRUTE_K nearest base rows per query (exact for u8/i8; FP32 for floats)."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        nq = q.shape[0]
+        ids = np.empty((nq, BRUTE_K), np.int32)
+        ds = np.empty((nq, BRUTE_K), np.float32)
+        check(lib().pk_brute_force(self.h, _p(q), nq, _p(ids), _p(ds)))
+        return ids, ds
+
+    def sync(self) -> None:
+        check(lib().pk_sync(self.h))
+
+    def device_error(self) -> int:
+        f = C.c_int()
+        check(lib().pk_device_error(self.h, C.byref(f)))
+        return f.value
+
+    def timer_begin(self) -> None:
+        check(lib().pk_timer_begin(self.h))
+
+    def timer_end(self) -> float:
+        ms = C.c_float()
+        check(lib().pk_timer_end(self.h, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,530 @@
+// index.hip -- the C-ABI (include/pickle_gpu.h): HBM-resident index state,
+// graph loading / saving, and the search entry points.
+//
+// Reference: HNSWGraph (include/hnsw.hpp:23-220) and HNSWLayer
+// (include/hnsw_layer.hpp:57-465) own the same state on the host; here it
+// lives in device memory laid out for the kernels:
+//   base        capacity rows x vstride bytes, by external id (m_data)
+//   layer l     adj   n_l * D_l int32 (+ slots read past the last row)
+//               deg   n_l int32, heur n_l int16, dist n_l * D_l float
+//               ext   n_l int32 (internal -> external id)
+//               vec   n_l rows x vstride (the layer's own copy, m_data)
+//               down  n_l int32: internal id of the node one layer below
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pickle_gpu.h"
+#include "index_impl.hpp"
+#include "search.hpp"
+
+using namespace pk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(PK_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+size_t elem_size(int dtype) {
+    switch (dtype) {
+        case PK_UINT8:
+        case PK_INT8: return 1;
+        case PK_FLOAT16: return 2;
+        case PK_FLOAT32: return 4;
+        default: return 0;
+    }
+}
+
+__global__ void k_gather_rows(uint8_t* dst, const uint8_t* base, const int32_t* ext, const uint8_t* has, int64_t n,
+                              int vstride, int64_t base_rows) {
+    int64_t row = (int64_t)blockIdx.x * (blockDim.x / 16) + threadIdx.x / 16;
+    int c = threadIdx.x % 16;
+    if (row >= n) return;
+    int chunks = vstride / 16;
+    int32_t e = ext[row];
+    bool ok = has[row] != 0 && e >= 0 && e < base_rows;
+    uint4* d = reinterpret_cast<uint4*>(dst + row * (int64_t)vstride);
+    const uint4* s = reinterpret_cast<const uint4*>(base + (int64_t)(ok ? e : 0) * vstride);
+    for (int i = c; i < chunks; i += 16) d[i] = ok ? s[i] : make_uint4(0, 0, 0, 0);
+}
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) return fail(PK_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    return PK_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_layer(Layer& L) {
+    dfree(L.adj);
+    dfree(L.deg);
+    dfree(L.heur);
+    dfree(L.dist);
+    dfree(L.ext);
+    dfree(L.vec);
+    dfree(L.down);
+    dfree(L.has);
+    L = Layer{};
+}
+
+}  // namespace
+
+namespace pk {
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace pk
+
+extern "C" {
+
+const char* pk_last_error(void) { return g_err.c_str(); }
+
+int pk_version(void) { return 100; }
+
+int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capacity, pk_index** out) {
+    if (!out) return fail(PK_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (elem_size(dtype) == 0) return fail(PK_ERR_ARG, "unsupported dtype %d", dtype);
+    if (metric != PK_L2 && metric != PK_IP) return fail(PK_ERR_ARG, "unsupported metric %d (L2=1, IP=2)", metric);
+    if (dim < 1) return fail(PK_ERR_ARG, "dim must be positive");
+    if (M < 2 || M > 16383) return fail(PK_ERR_ARG, "M out of range");
+    if (capacity < 1 || capacity > INT32_MAX) return fail(PK_ERR_ARG, "capacity out of range");
+    auto* h = new pk_index();
+    h->dtype = dtype;
+    h->metric = metric;
+    h->dim = dim;
+    h->M = M;
+    h->efc = efc;
+    h->capacity = capacity;
+    h->qbytes = (int)(dim * elem_size(dtype));
+    h->vstride = (h->qbytes + 15) / 16 * 16;
+    h->size_class = size_class_for(dtype, dim, h->vstride);
+    if (h->size_class < 0) {
+        delete h;
+        return fail(PK_ERR_ARG, "dim %d too large for this build (max 512 bytes per int row, 271 floats)", dim);
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        delete h;
+        return fail(PK_ERR_HIP, "stream / event creation failed");
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) h->num_cus = prop.multiProcessorCount;
+    if (h->num_cus <= 0) h->num_cus = 256;
+    *out = h;
+    return PK_OK;
+}
+
+int pk_index_destroy(pk_index* h) {
+    if (!h) return PK_OK;
+    (void)hipStreamSynchronize(h->stream);
+    for (auto& L : h->layers) free_layer(L);
+    dfree(h->base);
+    dfree(h->cglob);
+    dfree(h->vglob);
+    dfree(h->qcounter);
+    dfree(h->err);
+    for (auto& b : h->io) dfree(b.p);
+    (void)hipEventDestroy(h->ev0);
+    (void)hipEventDestroy(h->ev1);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PK_OK;
+}
+
+int pk_index_set_base(pk_index* h, const void* rows, int64_t first, int64_t n) {
+    if (!h || (!rows && n > 0)) return fail(PK_ERR_ARG, "null argument");
+    if (first < 0 || n < 0 || first + n > h->capacity) return fail(PK_ERR_ARG, "rows [%lld, %lld) outside capacity %lld",
+                                                                   (long long)first, (long long)(first + n),
+                                                                   (long long)h->capacity);
+    if (!h->base) {
+        int rc = dalloc(&h->base, (size_t)h->capacity * h->vstride);
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(h->base, 0, (size_t)h->capacity * h->vstride, h->stream));
+    }
+    if (n > 0)
+        HIPCHK(hipMemcpy2DAsync(h->base + (size_t)first * h->vstride, h->vstride, rows, h->qbytes, h->qbytes, n,
+                                hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->base_rows = std::max(h->base_rows, first + n);
+    return PK_OK;
+}
+
+int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_len, const int32_t* ext,
+                        const int32_t* degree, const int16_t* heur, const int32_t* adj, const float* dist,
+                        const uint8_t* has_vector) {
+    if (!h || !ext || !degree || !adj || !has_vector) return fail(PK_ERR_ARG, "null argument");
+    if (level < 0 || level >= PK_MAX_LEVELS) return fail(PK_ERR_ARG, "level %d out of range", level);
+    if (n < 0 || n > h->capacity || D < 1 || adj_len < n * D) return fail(PK_ERR_ARG, "bad layer shape");
+    if (!h->base) return fail(PK_ERR_STATE, "set the base rows before loading layers");
+    Layer& L = h->layers[level];
+    free_layer(L);
+    L.n = n;
+    L.D = D;
+    L.adj_len = adj_len;
+    const int64_t pad = 64;  // a chunk may read up to one wave past the degree
+    int rc;
+    if ((rc = dalloc(&L.adj, (size_t)(adj_len + pad))) || (rc = dalloc(&L.deg, (size_t)n)) ||
+        (rc = dalloc(&L.heur, (size_t)n)) || (rc = dalloc(&L.dist, (size_t)n * D)) ||
+        (rc = dalloc(&L.ext, (size_t)n)) || (rc = dalloc(&L.vec, (size_t)n * h->vstride)) ||
+        (rc = dalloc(&L.has, (size_t)n)) || (rc = dalloc(&L.down, (size_t)n)))
+        return rc;
+    HIPCHK(hipMemsetAsync(L.adj + adj_len, 0xff, pad * sizeof(int32_t), h->stream));
+    HIPCHK(hipMemcpyAsync(L.adj, adj, adj_len * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(L.deg, degree, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    if (heur)
+        HIPCHK(hipMemcpyAsync(L.heur, heur, n * sizeof(int16_t), hipMemcpyHostToDevice, h->stream));
+    else
+        HIPCHK(hipMemsetAsync(L.heur, 0, n * sizeof(int16_t), h->stream));
+    if (dist)
+        HIPCHK(hipMemcpyAsync(L.dist, dist, (size_t)n * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(L.ext, ext, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(L.has, has_vector, n, hipMemcpyHostToDevice, h->stream));
+    if (n > 0) {
+        int rows_per_block = 16;
+        int grid = (int)((n + rows_per_block - 1) / rows_per_block);
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, h->stream, L.vec, h->base, L.ext, L.has, n,
+                           h->vstride, h->capacity);
+        HIPCHK(hipGetLastError());
+    }
+    L.h_ext.assign(ext, ext + n);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    L.loaded = true;
+    h->ready = false;
+    return PK_OK;
+}
+
+int pk_index_set_enter_level(pk_index* h, int enter_level) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (enter_level < 0 || enter_level >= PK_MAX_LEVELS) return fail(PK_ERR_ARG, "enter level out of range");
+    for (int l = 0; l <= enter_level; l++)
+        if (!h->layers[l].loaded || h->layers[l].n == 0)
+            return fail(PK_ERR_STATE, "layer %d is not loaded (or empty) below the enter level", l);
+    // down maps: GetEntInID(GetExtID(v, l), l - 1) (include/hnsw.hpp:515-521)
+    std::vector<int32_t> ext2in((size_t)h->capacity, -1), down;
+    for (int l = 1; l <= enter_level; l++) {
+        Layer& lo = h->layers[l - 1];
+        Layer& up = h->layers[l];
+        std::fill(ext2in.begin(), ext2in.end(), -1);
+        for (int64_t i = 0; i < lo.n; i++) {
+            int32_t e = lo.h_ext[i];
+            if (e >= 0 && e < h->capacity) ext2in[e] = (int32_t)i;
+        }
+        down.assign(up.n, 0);
+        for (int64_t i = 0; i < up.n; i++) {
+            int32_t e = up.h_ext[i];
+            int32_t d = (e >= 0 && e < h->capacity) ? ext2in[e] : -1;
+            if (d < 0) return fail(PK_ERR_ARG, "node %lld of layer %d (ext %d) is missing from layer %d", (long long)i, l,
+                                   e, l - 1);
+            down[i] = d;
+        }
+        HIPCHK(hipMemcpy(up.down, down.data(), up.n * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    h->enter_level = enter_level;
+    h->ready = true;
+    return PK_OK;
+}
+
+int pk_index_info(pk_index* h, int* nlayers, int* enter_level, int64_t* sizes, int* degrees) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    int nl = 0;
+    for (int l = 0; l < PK_MAX_LEVELS; l++)
+        if (h->layers[l].loaded) nl = l + 1;
+    if (nlayers) *nlayers = nl;
+    if (enter_level) *enter_level = h->enter_level;
+    for (int l = 0; l < nl; l++) {
+        if (sizes) sizes[l] = h->layers[l].n;
+        if (degrees) degrees[l] = h->layers[l].D;
+    }
+    return PK_OK;
+}
+
+int pk_index_read_layer(pk_index* h, int level, int32_t* ext, int32_t* degree, int16_t* heur, int32_t* adj,
+                        float* dist) {
+    if (!h || level < 0 || level >= PK_MAX_LEVELS || !h->layers[level].loaded)
+        return fail(PK_ERR_ARG, "no such layer");
+    Layer& L = h->layers[level];
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (ext) HIPCHK(hipMemcpy(ext, L.ext, L.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (degree) HIPCHK(hipMemcpy(degree, L.deg, L.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (heur) HIPCHK(hipMemcpy(heur, L.heur, L.n * sizeof(int16_t), hipMemcpyDeviceToHost));
+    if (adj) HIPCHK(hipMemcpy(adj, L.adj, (size_t)L.n * L.D * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (dist) HIPCHK(hipMemcpy(dist, L.dist, (size_t)L.n * L.D * sizeof(float), hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// search
+// ---------------------------------------------------------------------------
+
+static int ensure_scratch(pk_index* h, int waves) {
+    if (waves <= h->scratch_waves) return PK_OK;
+    dfree(h->cglob);
+    dfree(h->vglob);
+    h->scratch_waves = 0;
+    int rc;
+    if ((rc = dalloc(&h->cglob, (size_t)waves * h->cg_cap))) return rc;
+    if ((rc = dalloc(&h->vglob, (size_t)waves << h->vg_bits))) return rc;
+    HIPCHK(hipMemsetAsync(h->vglob, 0, ((size_t)waves << h->vg_bits) * sizeof(unsigned long long), h->stream));
+    if (!h->qcounter && (rc = dalloc(&h->qcounter, 1))) return rc;
+    if (!h->err && (rc = dalloc(&h->err, 1))) return rc;
+    h->scratch_waves = waves;
+    h->next_epoch = 1;
+    return PK_OK;
+}
+
+int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                        int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                        float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (!h->ready) return fail(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
+    if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return fail(PK_ERR_ARG, "bad buffers");
+    if (k_stride < 1) return fail(PK_ERR_ARG, "k_stride must be positive");
+    if (!d_k && (k_all < 1 || k_all > k_stride)) return fail(PK_ERR_ARG, "k must be in [1, k_stride]");
+    if (!d_ef && ef_all < 1) return fail(PK_ERR_ARG, "ef must be positive");
+    if (ef_max < 1) return fail(PK_ERR_ARG, "ef_max must be positive");
+    if (nq == 0) return PK_OK;
+    if (nq > (int64_t)1 << 27) return fail(PK_ERR_CAPACITY, "at most 2^27 queries per call");
+
+    SearchArgs a{};
+    for (int l = 0; l <= h->enter_level; l++) {
+        const Layer& L = h->layers[l];
+        a.layers[l] = LayerDev{L.adj, L.deg, L.vec, L.down, L.n, L.adj_len, L.D, 0};
+    }
+    a.ext0 = h->layers[0].ext;
+    a.enter_level = h->enter_level;
+    a.dim = h->dim;
+    a.vstride = h->vstride;
+    a.qbytes = h->qbytes;
+    a.queries = static_cast<const uint8_t*>(d_queries);
+    a.nq = nq;
+    a.k = d_k;
+    a.ef = d_ef;
+    a.k_all = k_all;
+    a.ef_all = ef_all;
+    a.k_stride = k_stride;
+    a.out_ids = d_out_ids;
+    a.out_dists = d_out_dists;
+    a.out_count = d_out_count;
+    a.ctr = d_counters;
+
+    // LDS plan per wave: result heap, candidate heap, visited table,
+    // candidate id / distance lists, query row
+    const SearchTuning& t = h->tuning;
+    int wcap = (ef_max + 2 + 7) / 8 * 8;
+    int ccap = t.ccap > 0 ? t.ccap : (ef_max <= 64 ? 512 : 1024);
+    int vbits = t.vbits > 0 ? t.vbits : (ef_max <= 64 ? 11 : (ef_max <= 256 ? 12 : 13));
+    size_t per_wave = (size_t)wcap * 8 + (size_t)ccap * 8 + ((size_t)4 << vbits) + 64 * 4 * 2 + h->vstride;
+    per_wave = (per_wave + 15) / 16 * 16;
+    size_t lds = per_wave * (kSearchBlock / 64);
+    if (lds > 160 * 1024) return fail(PK_ERR_CAPACITY, "ef_max %d needs %zu bytes of LDS per workgroup", ef_max, lds);
+    int bpc = 0;
+    if (search_occupancy(h->dtype, h->metric, h->size_class, lds, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    if (t.blocks_per_cu > 0) bpc = std::min(bpc, t.blocks_per_cu);
+    int64_t want = (nq + (kSearchBlock / 64) - 1) / (kSearchBlock / 64);
+    int grid = (int)std::min<int64_t>(want, (int64_t)h->num_cus * bpc);
+    int waves = grid * (kSearchBlock / 64);
+    int rc = ensure_scratch(h, waves);
+    if (rc) return rc;
+
+    uint64_t span = (uint64_t)nq * 16 + 16;
+    if ((uint64_t)h->next_epoch + span >= 0xffffffffull) {
+        HIPCHK(hipMemsetAsync(h->vglob, 0, ((size_t)h->scratch_waves << h->vg_bits) * sizeof(unsigned long long),
+                              h->stream));
+        h->next_epoch = 1;
+    }
+    a.epoch_base = h->next_epoch;
+    h->next_epoch += (uint32_t)span;
+    a.cglob = h->cglob;
+    a.vglob = h->vglob;
+    a.cg_cap = h->cg_cap;
+    a.vg_bits = h->vg_bits;
+    a.wcap = wcap;
+    a.ccap = ccap;
+    a.vbits = vbits;
+    a.lds_per_wave = (int)per_wave;
+    a.qcounter = h->qcounter;
+    a.err = h->err;
+    HIPCHK(hipMemsetAsync(h->err, 0, sizeof(int32_t), h->stream));
+    h->last_grid = grid;
+    h->last_lds = lds;
+    HIPCHK(launch_search(a, h->dtype, h->metric, h->size_class, grid, lds, h->stream));
+    return PK_OK;
+}
+
+int pk_search_device(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                     int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
+                     int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (d_ef) return fail(PK_ERR_ARG, "per-query ef on device needs pk_search_device_ex (ef_max)");
+    return pk_search_device_ex(h, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_all, k_stride, d_out_ids, d_out_dists,
+                               d_out_count, d_counters);
+}
+
+int pk_device_error(pk_index* h, int* flags) {
+    if (!h || !h->err) return fail(PK_ERR_STATE, "no search has run");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(flags, h->err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+
+static int io_buf(pk_index* h, int slot, size_t bytes, void** p) {
+    auto& b = h->io[slot];
+    if (b.bytes < bytes) {
+        dfree(b.p);
+        b.bytes = 0;
+        int rc = dalloc(reinterpret_cast<uint8_t**>(&b.p), bytes);
+        if (rc) return rc;
+        b.bytes = bytes;
+    }
+    *p = b.p;
+    return PK_OK;
+}
+
+int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, const int32_t* ef, int32_t k_all,
+              int32_t ef_all, int32_t k_stride, int32_t* out_ids, float* out_dists, int32_t* out_count,
+              pk_query_counters* counters) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (nq < 0 || (nq > 0 && (!queries || !out_ids || !out_dists))) return fail(PK_ERR_ARG, "bad buffers");
+    if (nq == 0) return PK_OK;
+    int ef_max = ef_all;
+    if (ef) {
+        ef_max = 1;
+        for (int64_t i = 0; i < nq; i++) {
+            if (ef[i] < 1) return fail(PK_ERR_ARG, "ef[%lld] < 1", (long long)i);
+            ef_max = std::max(ef_max, ef[i]);
+        }
+    }
+    if (k)
+        for (int64_t i = 0; i < nq; i++)
+            if (k[i] < 1 || k[i] > k_stride) return fail(PK_ERR_ARG, "k[%lld] outside [1, k_stride]", (long long)i);
+    void *dq, *dk = nullptr, *def = nullptr, *dids, *dd, *dc, *dctr = nullptr;
+    int rc;
+    if ((rc = io_buf(h, 0, (size_t)nq * h->qbytes, &dq)) || (rc = io_buf(h, 3, (size_t)nq * k_stride * 4, &dids)) ||
+        (rc = io_buf(h, 4, (size_t)nq * k_stride * 4, &dd)) || (rc = io_buf(h, 5, (size_t)nq * 4, &dc)))
+        return rc;
+    if (k && (rc = io_buf(h, 1, (size_t)nq * 4, &dk))) return rc;
+    if (ef && (rc = io_buf(h, 2, (size_t)nq * 4, &def))) return rc;
+    if (counters && (rc = io_buf(h, 6, (size_t)nq * sizeof(pk_query_counters), &dctr))) return rc;
+    HIPCHK(hipMemcpyAsync(dq, queries, (size_t)nq * h->qbytes, hipMemcpyHostToDevice, h->stream));
+    if (k) HIPCHK(hipMemcpyAsync(dk, k, (size_t)nq * 4, hipMemcpyHostToDevice, h->stream));
+    if (ef) HIPCHK(hipMemcpyAsync(def, ef, (size_t)nq * 4, hipMemcpyHostToDevice, h->stream));
+    rc = pk_search_device_ex(h, dq, nq, (const int32_t*)dk, (const int32_t*)def, k_all, ef_all, ef_max, k_stride,
+                             (int32_t*)dids, (float*)dd, (int32_t*)dc, (pk_query_counters*)dctr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out_ids, dids, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(out_dists, dd, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost, h->stream));
+    std::vector<int32_t> cnt;
+    int32_t* cdst = out_count;
+    if (!cdst) {
+        cnt.resize(nq);
+        cdst = cnt.data();
+    }
+    HIPCHK(hipMemcpyAsync(cdst, dc, (size_t)nq * 4, hipMemcpyDeviceToHost, h->stream));
+    if (counters)
+        HIPCHK(hipMemcpyAsync(counters, dctr, (size_t)nq * sizeof(pk_query_counters), hipMemcpyDeviceToHost, h->stream));
+    int32_t errflag = 0;
+    HIPCHK(hipMemcpyAsync(&errflag, h->err, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (errflag) {
+        int64_t bad = 0;
+        for (int64_t i = 0; i < nq; i++) bad += cdst[i] < 0;
+        return fail(PK_ERR_DEVICE, "%lld queries failed on the device (flags %d: 1 = scratch capacity, 2 = bad k/ef)",
+                    (long long)bad, errflag);
+    }
+    return PK_OK;
+}
+
+int pk_wall_clock_khz(int* khz) {
+    if (!khz) return fail(PK_ERR_ARG, "null argument");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev));
+    return PK_OK;
+}
+
+int pk_sync(pk_index* h) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PK_OK;
+}
+
+int pk_device_alloc(void** p, size_t bytes) {
+    if (!p) return fail(PK_ERR_ARG, "null argument");
+    return dalloc(reinterpret_cast<uint8_t**>(p), bytes);
+}
+int pk_device_free(void* p) {
+    if (p) HIPCHK(hipFree(p));
+    return PK_OK;
+}
+int pk_copy_to_device(void* d, const void* s, size_t bytes) {
+    HIPCHK(hipMemcpy(d, s, bytes, hipMemcpyHostToDevice));
+    return PK_OK;
+}
+int pk_copy_to_host(void* d, const void* s, size_t bytes) {
+    HIPCHK(hipMemcpy(d, s, bytes, hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+
+int pk_timer_begin(pk_index* h) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    return PK_OK;
+}
+int pk_timer_end(pk_index* h, float* ms) {
+    if (!h || !ms) return fail(PK_ERR_ARG, "null argument");
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    HIPCHK(hipEventElapsedTime(ms, h->ev0, h->ev1));
+    return PK_OK;
+}
+
+int pk_set_tuning(pk_index* h, int ccap, int vbits, int blocks_per_cu) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (ccap < 0 || ccap > 16384 || (vbits != 0 && (vbits < 4 || vbits > 15)) || blocks_per_cu < 0)
+        return fail(PK_ERR_ARG, "bad tuning (ccap <= 16384, vbits 0 or 4..15)");
+    h->tuning.ccap = ccap;
+    h->tuning.vbits = vbits;
+    h->tuning.blocks_per_cu = blocks_per_cu;
+    return PK_OK;
+}
+
+int pk_last_launch(pk_index* h, int* grid, int* lds_bytes) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (grid) *grid = h->last_grid;
+    if (lds_bytes) *lds_bytes = (int)h->last_lds;
+    return PK_OK;
+}
+
+}  // extern "C"

@@ -149,8 +149,8 @@ int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const in
  * was exceeded, 2 = a query had k or ef out of range).  Waits for the stream. */
 int pk_device_error(pk_index *h, int *flags);
 
-/* Search launch shape: LDS candidate-heap entries, log2 of LDS visited slots
- * and a cap on resident workgroups per CU (0 = automatic).  No reference
+/* Search launch shape: LDS candidate-heap entries, log2 of the slots of each
+ * wave's visited table and a cap on resident workgroups per CU (0 = automatic).  No reference
  * counterpart (the reference has no device launch); set by the autotuner. */
 int pk_set_tuning(pk_index *h, int ccap, int vbits, int blocks_per_cu);
 

@@ -1,18 +1,25 @@
-// heap.hpp -- pickle's binary heaps, executed by a whole wave.
+// heap.hpp -- pickle's binary heaps, operated by a whole wave.
 //
 // Reference: EntryHeap (include/queue.hpp:99-289).  The candidate and
 // result heaps decide which of several equal-distance entries is expanded
-// or evicted first, so exact results need the same array layout: insert
-// appends and sifts up while the parent strictly ranks below the new entry
-// (queue.hpp:119-125, 204-216); pop moves the last entry to the root and
-// sifts down, taking the left child first and the right child only when it
-// is strictly better (queue.hpp:127-150, 230-236).
+// or evicted first, so exact results need the reference's array layout:
+// insert appends and sifts up while the parent strictly ranks below the new
+// entry (queue.hpp:119-125, 204-216); pop moves the last entry to the root
+// and sifts down, taking the left child first and the right child only when
+// it is strictly better (queue.hpp:127-150, 230-236).
 //
-// Every lane runs the same (uniform) control flow; values read from memory
-// are made scalar with readfirstlane, and only lane 0 stores.  The array
-// lives in LDS up to `lcap` entries; deeper entries spill to a per-wave
+// The array lives in LDS up to `lcap` entries and continues in a per-wave
 // region of device memory (the candidate heap is unbounded in the
-// reference, queue.hpp:191-198).
+// reference, queue.hpp:191-198).  The sequential steps are replaced by
+// wave-parallel ones that leave the same array:
+//   push_par  lane j reads ancestor j of the new slot; a ballot finds the
+//             first ancestor the entry does not pass; the passed ancestors
+//             shift down one level and the entry lands below the stop.
+//   pop_par   every internal node's preferred child (left unless the right
+//             one is strictly better) is found in parallel, then the path
+//             from the root is followed with v_readlane until the moved
+//             entry no longer ranks below the child; the path shifts up.
+//   pop       the sequential sift-down (for heaps too large for pop_par).
 #pragma once
 #include "wave.hpp"
 
@@ -25,6 +32,9 @@ __device__ __forceinline__ uint64_t pack_entry(float d, int id) {
 __device__ __forceinline__ float entry_dist(uint64_t e) { return __uint_as_float((uint32_t)e); }
 __device__ __forceinline__ int entry_id(uint64_t e) { return (int)(e >> 32); }
 
+// internal nodes handled per lane by pop_par: 64 * kPopSlots >= len / 2
+constexpr int kPopSlots = 5;
+
 template <bool IS_MAX, bool SPILL>
 struct WaveHeap {
     uint64_t* lds;   // lcap entries
@@ -36,40 +46,48 @@ struct WaveHeap {
     // true when x ranks below y (MaxFirst / MinFirst, queue.hpp:30-40)
     __device__ static __forceinline__ bool below(float x, float y) { return IS_MAX ? (x < y) : (x > y); }
 
-    __device__ __forceinline__ uint64_t get(int i) const {
-        uint64_t v;
-        if (!SPILL || i < lcap)
-            v = lds[i];
-        else
-            v = glob[i - lcap];
-        return uni64(v);
+    // per-lane access (any index); device-memory entries written by lane 0
+    // are read from L2, not from a possibly stale L1 line
+    __device__ __forceinline__ uint64_t raw_get(int i) const {
+        if (!SPILL || i < lcap) return lds[i];
+        return __hip_atomic_load(&glob[i - lcap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __device__ __forceinline__ void raw_put(int i, uint64_t e) {
+        if (!SPILL || i < lcap)
+            lds[i] = e;
+        else
+            glob[i - lcap] = e;
+    }
+
+    // uniform access (same index on every lane)
+    __device__ __forceinline__ uint64_t get(int i) const { return uni64(raw_get(uni(i))); }
     __device__ __forceinline__ void put(int i, uint64_t e, bool leader) {
-        if (leader) {
-            if (!SPILL || i < lcap)
-                lds[i] = e;
-            else
-                glob[i - lcap] = e;
-        }
+        if (leader) raw_put(i, e);
     }
     __device__ __forceinline__ bool full() const { return len >= lcap + (SPILL ? gcap : 0); }
 
-    __device__ void push(uint64_t e, bool leader) {
-        int i = len++;
-        float d = entry_dist(e);
-        while (i > 0) {
-            int p = (i - 1) >> 1;
-            uint64_t pe = get(p);
-            if (below(entry_dist(pe), d)) {
-                put(i, pe, leader);
-                i = p;
-            } else {
-                break;
-            }
-        }
-        put(i, e, leader);
+    __device__ void push_par(uint64_t e, int lane, bool leader) {
+        const int i = uni(len);
+        len = i + 1;
+        const float d = entry_dist(e);
+        const int t = 31 - __builtin_clz((unsigned)(i + 1));  // number of ancestors
+        const int j = lane + 1;                              // this lane's ancestor
+        const bool act = j <= t;
+        const int aj = act ? ((i + 1) >> j) - 1 : 0;
+        const uint64_t v = act ? raw_get(aj) : 0;
+        const bool up = act && below(entry_dist(v), d);
+        const uint64_t stopm = ballot(act && !up);
+        const int stop = stopm ? (int)__builtin_ctzll(stopm) + 1 : t + 1;
+        __builtin_amdgcn_wave_barrier();
+        if (act && j < stop) raw_put(((i + 1) >> (j - 1)) - 1, v);
+        if (leader) raw_put(((i + 1) >> (stop - 1)) - 1, e);
+        // device-memory slots written by several lanes: complete the stores
+        // before any lane reads them back
+        if (SPILL && i >= lcap) __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
     }
 
+    // sequential sift-down, for any size
     __device__ void pop(bool leader) {
         uint64_t x = get(len - 1);
         len -= 1;
@@ -100,6 +118,61 @@ struct WaveHeap {
             i = best;
         }
         put(i, x, leader);
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // parallel sift-down; needs len <= 128 * kPopSlots
+    __device__ void pop_par(int lane, bool leader) {
+        const int n = uni(len) - 1;
+        const uint64_t x = get(n);
+        len = n;
+        if (n == 0) return;
+        const float xd = entry_dist(x);
+        int pc[kPopSlots];
+        uint32_t pvl[kPopSlots], pvh[kPopSlots];
+#pragma unroll
+        for (int s = 0; s < kPopSlots; s++) {
+            const int node = lane + kWave * s;
+            const int l = 2 * node + 1, r = l + 1;
+            int c = -1;
+            uint64_t cv = 0;
+            if (l < n) {
+                uint64_t lv = raw_get(l);
+                c = l;
+                cv = lv;
+                if (r < n) {
+                    uint64_t rv = raw_get(r);
+                    if (below(entry_dist(lv), entry_dist(rv))) {
+                        c = r;
+                        cv = rv;
+                    }
+                }
+            }
+            pc[s] = c;
+            pvl[s] = (uint32_t)cv;
+            pvh[s] = (uint32_t)(cv >> 32);
+        }
+        __builtin_amdgcn_wave_barrier();
+        int cur = 0;
+        for (;;) {
+            const int s = cur >> 6, ln = cur & 63;
+            int c = -1;
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int k = 0; k < kPopSlots; k++)
+                if (k == s) {
+                    c = readlane(pc[k], ln);
+                    lo = (uint32_t)readlane((int)pvl[k], ln);
+                    hi = (uint32_t)readlane((int)pvh[k], ln);
+                }
+            if (c < 0) break;
+            const uint64_t cv = (uint64_t)lo | ((uint64_t)hi << 32);
+            if (!below(xd, entry_dist(cv))) break;
+            put(cur, cv, leader);
+            cur = c;
+        }
+        put(cur, x, leader);
+        __builtin_amdgcn_wave_barrier();
     }
 
     __device__ __forceinline__ uint64_t top() const { return get(0); }

@@ -23,6 +23,7 @@
 #include "../../include/pickle_gpu.h"
 #include "index_impl.hpp"
 #include "search.hpp"
+#include "visited.hpp"
 
 using namespace pk;
 
@@ -153,7 +154,9 @@ int pk_index_destroy(pk_index* h) {
     for (auto& L : h->layers) free_layer(L);
     dfree(h->base);
     dfree(h->cglob);
-    dfree(h->vglob);
+    dfree(h->vtab);
+    dfree(h->vlog);
+    if (h->d_layers) (void)hipFree(h->d_layers);
     dfree(h->qcounter);
     dfree(h->err);
     for (auto& b : h->io) dfree(b.p);
@@ -252,6 +255,13 @@ int pk_index_set_enter_level(pk_index* h, int enter_level) {
         }
         HIPCHK(hipMemcpy(up.down, down.data(), up.n * sizeof(int32_t), hipMemcpyHostToDevice));
     }
+    LayerDev desc[PK_MAX_LEVELS] = {};
+    for (int l = 0; l <= enter_level; l++) {
+        const Layer& L = h->layers[l];
+        desc[l] = LayerDev{L.adj, L.deg, L.vec, L.down, L.n, L.adj_len, L.D, 0};
+    }
+    if (!h->d_layers) HIPCHK(hipMalloc(&h->d_layers, sizeof(desc)));
+    HIPCHK(hipMemcpy(h->d_layers, desc, sizeof(desc), hipMemcpyHostToDevice));
     h->enter_level = enter_level;
     h->ready = true;
     return PK_OK;
@@ -289,19 +299,26 @@ int pk_index_read_layer(pk_index* h, int level, int32_t* ext, int32_t* degree, i
 // search
 // ---------------------------------------------------------------------------
 
-static int ensure_scratch(pk_index* h, int waves) {
-    if (waves <= h->scratch_waves) return PK_OK;
+static int ensure_scratch(pk_index* h, int waves, int vbits) {
+    if (waves <= h->scratch_waves && vbits <= h->scratch_vbits) return PK_OK;
+    waves = std::max(waves, h->scratch_waves);
+    vbits = std::max(vbits, h->scratch_vbits);
+    HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h->cglob);
-    dfree(h->vglob);
+    dfree(h->vtab);
+    dfree(h->vlog);
     h->scratch_waves = 0;
+    h->scratch_vbits = 0;
     int rc;
+    size_t slots = (size_t)waves << vbits;
     if ((rc = dalloc(&h->cglob, (size_t)waves * h->cg_cap))) return rc;
-    if ((rc = dalloc(&h->vglob, (size_t)waves << h->vg_bits))) return rc;
-    HIPCHK(hipMemsetAsync(h->vglob, 0, ((size_t)waves << h->vg_bits) * sizeof(unsigned long long), h->stream));
+    if ((rc = dalloc(&h->vtab, slots))) return rc;
+    if ((rc = dalloc(&h->vlog, slots))) return rc;
+    HIPCHK(hipMemsetAsync(h->vtab, 0, slots * sizeof(uint32_t), h->stream));
     if (!h->qcounter && (rc = dalloc(&h->qcounter, 1))) return rc;
     if (!h->err && (rc = dalloc(&h->err, 1))) return rc;
     h->scratch_waves = waves;
-    h->next_epoch = 1;
+    h->scratch_vbits = vbits;
     return PK_OK;
 }
 
@@ -319,10 +336,7 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     if (nq > (int64_t)1 << 27) return fail(PK_ERR_CAPACITY, "at most 2^27 queries per call");
 
     SearchArgs a{};
-    for (int l = 0; l <= h->enter_level; l++) {
-        const Layer& L = h->layers[l];
-        a.layers[l] = LayerDev{L.adj, L.deg, L.vec, L.down, L.n, L.adj_len, L.D, 0};
-    }
+    a.layers = static_cast<const LayerDev*>(h->d_layers);
     a.ext0 = h->layers[0].ext;
     a.enter_level = h->enter_level;
     a.dim = h->dim;
@@ -340,13 +354,16 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     a.out_count = d_out_count;
     a.ctr = d_counters;
 
-    // LDS plan per wave: result heap, candidate heap, visited table,
-    // candidate id / distance lists, query row
+    // LDS plan per wave: result heap, candidate heap (deeper levels spill to
+    // device memory), candidate id / distance lists, query row.  The visited table lives in device memory,
+    // sized for the visits ef implies (a full table fails the query).
     const SearchTuning& t = h->tuning;
-    int wcap = (ef_max + 2 + 7) / 8 * 8;
+    int wcap = result_capacity_for(ef_max);
+    if (wcap < 0) return fail(PK_ERR_ARG, "ef_max %d above the largest supported ef (630)", ef_max);
     int ccap = t.ccap > 0 ? t.ccap : (ef_max <= 64 ? 512 : 1024);
-    int vbits = t.vbits > 0 ? t.vbits : (ef_max <= 64 ? 11 : (ef_max <= 256 ? 12 : 13));
-    size_t per_wave = (size_t)wcap * 8 + (size_t)ccap * 8 + ((size_t)4 << vbits) + 64 * 4 * 2 + h->vstride;
+    int vbits = t.vbits > 0 ? t.vbits : (ef_max <= 32 ? 14 : (ef_max <= 128 ? 15 : (ef_max <= 512 ? 16 : 17)));
+    if (h->capacity > kVisitMaxId) return fail(PK_ERR_CAPACITY, "search supports at most %lld ids", (long long)kVisitMaxId);
+    size_t per_wave = (size_t)wcap * 8 + (size_t)ccap * 8 + 64 * 4 * 2 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
     size_t lds = per_wave * (kSearchBlock / 64);
     if (lds > 160 * 1024) return fail(PK_ERR_CAPACITY, "ef_max %d needs %zu bytes of LDS per workgroup", ef_max, lds);
@@ -356,24 +373,15 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     int64_t want = (nq + (kSearchBlock / 64) - 1) / (kSearchBlock / 64);
     int grid = (int)std::min<int64_t>(want, (int64_t)h->num_cus * bpc);
     int waves = grid * (kSearchBlock / 64);
-    int rc = ensure_scratch(h, waves);
+    int rc = ensure_scratch(h, waves, vbits);
     if (rc) return rc;
-
-    uint64_t span = (uint64_t)nq * 16 + 16;
-    if ((uint64_t)h->next_epoch + span >= 0xffffffffull) {
-        HIPCHK(hipMemsetAsync(h->vglob, 0, ((size_t)h->scratch_waves << h->vg_bits) * sizeof(unsigned long long),
-                              h->stream));
-        h->next_epoch = 1;
-    }
-    a.epoch_base = h->next_epoch;
-    h->next_epoch += (uint32_t)span;
     a.cglob = h->cglob;
-    a.vglob = h->vglob;
+    a.vtab = h->vtab;
+    a.vlog = h->vlog;
     a.cg_cap = h->cg_cap;
-    a.vg_bits = h->vg_bits;
+    a.vg_bits = h->scratch_vbits;
     a.wcap = wcap;
     a.ccap = ccap;
-    a.vbits = vbits;
     a.lds_per_wave = (int)per_wave;
     a.qcounter = h->qcounter;
     a.err = h->err;
@@ -512,8 +520,8 @@ int pk_timer_end(pk_index* h, float* ms) {
 
 int pk_set_tuning(pk_index* h, int ccap, int vbits, int blocks_per_cu) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
-    if (ccap < 0 || ccap > 16384 || (vbits != 0 && (vbits < 4 || vbits > 15)) || blocks_per_cu < 0)
-        return fail(PK_ERR_ARG, "bad tuning (ccap <= 16384, vbits 0 or 4..15)");
+    if (ccap < 0 || ccap > 16384 || (vbits != 0 && (vbits < 8 || vbits > 20)) || blocks_per_cu < 0)
+        return fail(PK_ERR_ARG, "bad tuning (ccap <= 16384, vbits 0 or 8..20)");
     h->tuning.ccap = ccap;
     h->tuning.vbits = vbits;
     h->tuning.blocks_per_cu = blocks_per_cu;

@@ -27,7 +27,7 @@ struct Layer {
 
 struct SearchTuning {
     int ccap = 0;           // candidate heap entries in LDS (0 = by ef)
-    int vbits = 0;          // log2 LDS visited slots (0 = by ef)
+    int vbits = 0;          // log2 slots of each wave's visited table (0 = by ef)
     int blocks_per_cu = 0;  // cap on resident workgroups per CU (0 = occupancy)
 };
 
@@ -53,15 +53,16 @@ struct pk_index {
     uint8_t* base = nullptr;
     int64_t base_rows = 0;
     pk::Layer layers[PK_MAX_LEVELS];
+    void* d_layers = nullptr;  // device copy of the kernels' layer descriptors
     int enter_level = 0;
     bool ready = false;
     // search scratch (per resident wave)
     uint64_t* cglob = nullptr;
-    unsigned long long* vglob = nullptr;
+    uint32_t* vtab = nullptr;
+    int32_t* vlog = nullptr;
     int scratch_waves = 0;
+    int scratch_vbits = 0;
     int cg_cap = 16384;
-    int vg_bits = 15;
-    uint32_t next_epoch = 1;
     int32_t* qcounter = nullptr;
     int32_t* err = nullptr;
     pk::SearchTuning tuning;

@@ -8,13 +8,14 @@
 // A persistent grid: every wave takes the next query index from an atomic
 // counter until the batch is drained (uneven ef / k per query balance
 // themselves).  Per hop the wave
-//   1. pops the closest candidate (heap in LDS, spill in device memory),
+//   1. pops the closest candidate (heap in LDS, deep levels in device memory),
 //   2. reads the adjacency row, one id per lane (coalesced 4-byte loads),
-//   3. tests and marks the ids in the exact visited set (LDS hash),
+//   3. tests and marks the ids in the exact visited set (one atomic per id),
 //   4. gathers the new rows (one lane group per row, 16-byte loads) and
 //      computes their distances,
 //   5. applies the reference's accept rule in adjacency order, updating the
-//      result and candidate heaps exactly as the reference's binary heaps.
+//      result and candidate heaps exactly as the reference's binary heaps
+//      (each push and result-heap pop is one wave-parallel step).
 // Step 5 is cut down with one test: when the result heap is full at the
 // start of a chunk, its top only shrinks, so rows at or beyond it are
 // rejected without touching the heaps.
@@ -73,27 +74,6 @@ __device__ void batch_distances(const DF& df, const uint8_t* vec, int vstride, c
     }
 }
 
-// Tests and marks one chunk of adjacency ids.  Returns the lanes whose id is
-// new, with an id repeated inside the chunk credited to its first lane (the
-// reference scans the row in order, include/hnsw.hpp:413-422).
-__device__ uint64_t visit_chunk(WaveVisited& vis, int id, bool valid, int& fail) {
-    bool present = valid ? vis.test(id) : true;
-    bool want = !present;
-    int r = want ? vis.insert(id) : 1;
-    if (ballot(r == 2)) fail = 1;
-    uint64_t newm = ballot(want && r == 0);
-    uint64_t dupm = ballot(want && r == 1);
-    while (dupm) {
-        int j = __builtin_ctzll(dupm);
-        int idj = readlane(id, j);
-        uint64_t m = ballot(want && id == idj);
-        int first = __builtin_ctzll(m);
-        newm = (newm & ~m) | (1ull << first);
-        dupm &= ~m;
-    }
-    return newm;
-}
-
 __device__ __forceinline__ int clamp_degree(const LayerDev& L, int v) {
     int deg = L.deg[v];
     int64_t room = L.adj_len - (int64_t)v * L.D;
@@ -135,19 +115,17 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
     uint8_t* my = smem + (size_t)w * A.lds_per_wave;
     uint64_t* Wl = reinterpret_cast<uint64_t*>(my);
     uint64_t* Cl = Wl + A.wcap;
-    int* vt = reinterpret_cast<int*>(Cl + A.ccap);
-    int* cl = vt + (1 << A.vbits);
+    int* cl = reinterpret_cast<int*>(Cl + A.ccap);
     float* cd = reinterpret_cast<float*>(cl + kWave);
     uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kWave);
 
     WaveVisited vis;
-    vis.tab = vt;
-    vis.bits = A.vbits;
-    vis.mask = (1u << A.vbits) - 1;
-    vis.gtab = A.vglob + ((size_t)gw << A.vg_bits);
-    vis.gbits = A.vg_bits;
-    vis.gmask = (1u << A.vg_bits) - 1;
-    vis.epoch = 0;
+    vis.tab = A.vtab + ((size_t)gw << A.vg_bits);
+    vis.log = A.vlog + ((size_t)gw << A.vg_bits);
+    vis.bits = A.vg_bits;
+    vis.mask = (1u << A.vg_bits) - 1;
+    vis.count = 0;
+    vis.tag = 0;
     uint64_t* cg = A.cglob + (size_t)gw * (size_t)A.cg_cap;
 
     const int g = lane % DF::LPC;
@@ -175,12 +153,9 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
 
         // ---- upper layers: SlideLayer (include/hnsw.hpp:446-494) ----
         for (int level = A.enter_level; level >= 1 && !fail; level--) {
-            const LayerDev& L = A.layers[level];
-            vis.reset(A.epoch_base + (uint32_t)q * 16u + (uint32_t)level, lane);
-            __builtin_amdgcn_wave_barrier();
+            const LayerDev L = A.layers[level];
             float cdist = one_distance(df, L, A.vstride, cur, cl, cd, lane);
-            if (leader && vis.insert(cur) == 2) fail = 1;
-            fail = readlane(fail, 0);
+            vis.mark(cur, leader, lane, fail);
             bool updated;
             do {
                 updated = false;
@@ -193,7 +168,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
                     bool valid = j < deg;
                     int id = valid ? L.adj[(int64_t)scan * L.D + j] : -1;
                     valid = valid && id >= 0 && (int64_t)id < L.n;
-                    uint64_t newm = visit_chunk(vis, id, valid, fail);
+                    uint64_t newm = vis.mark(id, valid, lane, fail);
                     if (newm == 0) continue;
                     dsu += __builtin_popcountll(newm);
                     float d = chunk_distances(df, L, A.vstride, newm, id, cl, cd, lane);
@@ -206,6 +181,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
                     }
                 }
             } while (updated && !fail);
+            vis.clear(lane);
             if (!fail) cur = L.down[cur];
         }
 
@@ -213,19 +189,16 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
         WaveHeap<true, false> W{Wl, Cl, A.wcap, 0, 0};
         WaveHeap<false, true> C{Cl, cg, A.ccap, A.cg_cap, 0};
         if (!fail) {
-            const LayerDev& L = A.layers[0];
-            vis.reset(A.epoch_base + (uint32_t)q * 16u, lane);
-            __builtin_amdgcn_wave_barrier();
+            const LayerDev L = A.layers[0];
             float d0 = one_distance(df, L, A.vstride, cur, cl, cd, lane);
-            W.push(pack_entry(d0, cur), leader);
-            C.push(pack_entry(d0, cur), leader);
-            if (leader && vis.insert(cur) == 2) fail = 1;
-            fail = readlane(fail, 0);
-            float wtop = d0;
+            W.push_par(pack_entry(d0, cur), lane, leader);
+            C.push_par(pack_entry(d0, cur), lane, leader);
+            vis.mark(cur, leader, lane, fail);
+            uint32_t wtop_bits = __float_as_uint(d0);  // bits of W.top()
             while (C.len > 0 && !fail) {
-                uint64_t te = C.top();
+                const uint64_t te = C.top();
                 C.pop(leader);
-                if (entry_dist(te) > wtop) break;  // hnsw.hpp:403
+                if (entry_dist(te) > __uint_as_float(wtop_bits)) break;  // hnsw.hpp:403
                 const int c = entry_id(te);
                 const int deg = uni(clamp_degree(L, c));
                 hop0++;
@@ -235,31 +208,32 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
                     bool valid = j < deg;
                     int id = valid ? L.adj[(int64_t)c * L.D + j] : -1;
                     valid = valid && id >= 0 && (int64_t)id < L.n;
-                    uint64_t newm = visit_chunk(vis, id, valid, fail);
+                    uint64_t newm = vis.mark(id, valid, lane, fail);
                     if (newm == 0) continue;
                     ds0 += __builtin_popcountll(newm);
                     float d = chunk_distances(df, L, A.vstride, newm, id, cl, cd, lane);
                     bool isnew = (newm >> lane) & 1;
+                    const float wtop = __uint_as_float(wtop_bits);
                     uint64_t cand = W.len >= ef ? ballot(isnew && d < wtop) : newm;
                     while (cand) {
                         int ln = __builtin_ctzll(cand);
                         cand &= cand - 1;
-                        float dv = readlanef(d, ln);
-                        int iv = readlane(id, ln);
-                        if (W.len < ef || wtop > dv) {  // hnsw.hpp:425-431
-                            uint64_t e = pack_entry(dv, iv);
-                            W.push(e, leader);
+                        const float dv = readlanef(d, ln);
+                        if (W.len < ef || __uint_as_float(wtop_bits) > dv) {  // hnsw.hpp:425-431
+                            const uint64_t e = pack_entry(dv, readlane(id, ln));
+                            W.push_par(e, lane, leader);
                             if (C.full()) {
                                 fail = 1;
                                 break;
                             }
-                            C.push(e, leader);
-                            if (W.len > ef) W.pop(leader);
-                            wtop = entry_dist(W.top());
+                            C.push_par(e, lane, leader);
+                            if (W.len > ef) W.pop_par(lane, leader);
+                            wtop_bits = (uint32_t)W.top();
                         }
                     }
                 }
             }
+            vis.clear(lane);
         }
 
         // ---- extractMinTopK + external ids (queue.hpp:265-286, hnsw.hpp:118-125) ----
@@ -268,14 +242,14 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
         int kk = 0;
         if (!fail) {
             kk = k < W.len ? k : W.len;
-            while (W.len > kk) W.pop(leader);
+            while (W.len > kk) W.pop_par(lane, leader);
             for (int i = 0; i < kk; i++) {
-                uint64_t e = W.top();
+                const uint64_t e = W.top();
                 if (leader) {
                     oid[kk - 1 - i] = A.ext0[entry_id(e)];
                     odist[kk - 1 - i] = entry_dist(e);
                 }
-                W.pop(leader);
+                W.pop_par(lane, leader);
             }
         }
         for (int i = kk + lane; i < A.k_stride; i += kWave) {
@@ -306,24 +280,23 @@ __global__ void k_fill_i32(int* p, int v, int n) {
     if (i < n) p[i] = v;
 }
 
+template <class DF, bool INT>
+static hipError_t launch_rs(const SearchArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_search<DF, INT>), dim3(grid), dim3(kSearchBlock), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int DT, int MET>
 static hipError_t launch_dt(const SearchArgs& a, int size_class, int grid, size_t lds, hipStream_t s) {
     constexpr bool INT = DT == DT_U8 || DT == DT_I8;
     if constexpr (INT) {
-        if (size_class == 0)
-            hipLaunchKernelGGL((k_search<IntDist<DT, MET, 1>, true>), dim3(grid), dim3(kSearchBlock), lds, s, a);
-        else if (size_class == 1)
-            hipLaunchKernelGGL((k_search<IntDist<DT, MET, 2>, true>), dim3(grid), dim3(kSearchBlock), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_search<IntDist<DT, MET, 4>, true>), dim3(grid), dim3(kSearchBlock), lds, s, a);
+        if (size_class == 0) return launch_rs<IntDist<DT, MET, 1>, true>(a, grid, lds, s);
+        if (size_class == 1) return launch_rs<IntDist<DT, MET, 2>, true>(a, grid, lds, s);
+        return launch_rs<IntDist<DT, MET, 4>, true>(a, grid, lds, s);
     } else {
-        if (size_class == 0)
-            hipLaunchKernelGGL((k_search<FloatDist<DT, MET, 8>, false>), dim3(grid), dim3(kSearchBlock), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_search<FloatDist<DT, MET, 16>, false>), dim3(grid), dim3(kSearchBlock), lds, s,
-                               a);
+        if (size_class == 0) return launch_rs<FloatDist<DT, MET, 8>, false>(a, grid, lds, s);
+        return launch_rs<FloatDist<DT, MET, 16>, false>(a, grid, lds, s);
     }
-    return hipGetLastError();
 }
 
 int size_class_for(int dtype, int dim, int vstride) {
@@ -338,6 +311,11 @@ int size_class_for(int dtype, int dim, int vstride) {
     if (nmain <= 8) return 0;
     if (nmain <= 16) return 1;
     return -1;
+}
+
+int result_capacity_for(int ef_max) {
+    int cap = (ef_max + 2 + 7) / 8 * 8;
+    return cap <= 2 * kWave * kPopSlots ? cap : -1;
 }
 
 hipError_t launch_search(const SearchArgs& a, int dtype, int metric, int size_class, int grid, size_t lds,

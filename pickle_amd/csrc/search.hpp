@@ -23,7 +23,7 @@ struct LayerDev {
 };
 
 struct SearchArgs {
-    LayerDev layers[PK_MAX_LEVELS];
+    const LayerDev* layers;  // device array, PK_MAX_LEVELS entries
     const int32_t* ext0;  // layer-0 internal -> external id
     int32_t enter_level;
     int32_t dim;
@@ -38,20 +38,20 @@ struct SearchArgs {
     float* out_dists;
     int32_t* out_count;
     pk_query_counters* ctr;
-    uint64_t* cglob;            // per-wave candidate-heap spill, cg_cap entries each
-    unsigned long long* vglob;  // per-wave visited spill, 2^vg_bits entries each
+    uint64_t* cglob;  // per-wave candidate-heap spill, cg_cap entries each
+    uint32_t* vtab;   // per-wave visited tables, 2^vg_bits slots each (zero between searches)
+    int32_t* vlog;    // per-wave filled-slot logs, 2^vg_bits entries each
     int32_t cg_cap;
     int32_t vg_bits;
-    uint32_t epoch_base;
-    int32_t wcap;   // result heap entries in LDS (>= ef + 1)
-    int32_t ccap;   // candidate heap entries in LDS
-    int32_t vbits;  // log2 of LDS visited slots
+    int32_t wcap;  // result heap entries in LDS (>= ef + 1)
+    int32_t ccap;  // candidate heap entries in LDS
     int32_t lds_per_wave;
     int32_t* qcounter;
     int32_t* err;
 };
 
 int size_class_for(int dtype, int dim, int vstride);
+int result_capacity_for(int ef_max);
 hipError_t launch_search(const SearchArgs& a, int dtype, int metric, int size_class, int grid, size_t lds,
                          hipStream_t s);
 hipError_t search_occupancy(int dtype, int metric, int size_class, size_t lds, int* blocks_per_cu);

@@ -1,89 +1,93 @@
-// visited.hpp -- exact per-query visited set: LDS hash table + device spill.
+// visited.hpp -- exact per-search visited set in device memory.
 //
-// Reference: ByteMap (include/visited.hpp:248-297) gives an exact answer
-// with one byte per node per thread.  Here each wave keeps an open-addressed
-// table of ids in LDS (linear probing, bounded probe window); an id whose
-// window is full goes to a per-wave table in device memory, in the manner
-// of the reference's HashBitmap (visited.hpp:17-101: a small direct table
-// plus an exact overflow set).  Device entries carry the search's epoch in
-// their high word, so a new search needs no clearing there: entries of
-// older epochs read as empty.  There are no false positives, so traversal
-// is the reference's.
+// Reference: ByteMap (include/visited.hpp:248-297) is exact with one byte
+// per node per thread.  Here each wave owns an open-addressed hash table of
+// 2^bits 32-bit slots in device memory plus a log of the slots it filled.
+// Test-and-mark is one atomic compare-and-swap per lane against an empty
+// slot, so a chunk of 64 adjacency ids costs one memory round trip; after a
+// search the wave zeroes exactly the slots in its log, so the table is empty
+// again for the next search without any per-search clearing of the table.
+//
+// A slot holds (tag << 27) | (id + 1): the tag is the chunk counter mod 32,
+// which tells a lane that finds its own id whether another lane of the same
+// chunk put it there (an id repeated within one adjacency row) or an
+// earlier chunk did.  Ids must be below 2^27 - 1.
 #pragma once
 #include "wave.hpp"
 
 namespace pk {
 
-constexpr int kVisitProbe = 16;
+constexpr uint32_t kVisitIdMask = (1u << 27) - 1;
+constexpr int64_t kVisitMaxId = (int64_t)kVisitIdMask - 1;
 
 struct WaveVisited {
-    int* tab;                  // LDS, `mask + 1` slots, -1 = empty
+    uint32_t* tab;  // 2^bits slots, all zero between searches
+    int32_t* log;   // 2^bits filled-slot indices
     uint32_t mask;
     int bits;
-    unsigned long long* gtab;  // device spill, `gmask + 1` slots
-    uint32_t gmask;
-    int gbits;
-    uint32_t epoch;
+    int count;      // slots filled by the current search (uniform)
+    uint32_t tag;   // chunk counter
 
-    __device__ __forceinline__ uint32_t h1(int id) const {
+    __device__ __forceinline__ uint32_t home(int id) const {
         return ((uint32_t)id * 0x9E3779B1u) >> (32 - bits);
     }
-    __device__ __forceinline__ uint32_t h2(int id) const {
-        return ((uint32_t)id * 0x85EBCA6Bu) >> (32 - gbits);
-    }
 
-    // all lanes call; empties the LDS table
-    __device__ void reset(uint32_t new_epoch, int lane) {
-        epoch = new_epoch;
-        int4* t4 = reinterpret_cast<int4*>(tab);
-        int n4 = (int)((mask + 1) >> 2);
-        for (int i = lane; i < n4; i += kWave) t4[i] = make_int4(-1, -1, -1, -1);
-    }
-
-    __device__ bool test(int id) const {
-        uint32_t h = h1(id);
-#pragma unroll 4
-        for (int p = 0; p < kVisitProbe; p++) {
-            int v = tab[(h + p) & mask];
-            if (v == id) return true;
-            if (v == -1) return false;
-        }
-        unsigned long long key = ((unsigned long long)epoch << 32) | (uint32_t)id;
-        uint32_t g = h2(id);
-        for (uint32_t p = 0; p <= gmask; p++) {
-            unsigned long long v = gtab[(g + p) & gmask];
-            if ((uint32_t)(v >> 32) != epoch) return false;
-            if (v == key) return true;
-        }
-        return false;
-    }
-
-    // 0: inserted by this lane, 1: already present (possibly inserted by a
-    // lane of the same call), 2: device table full
-    __device__ int insert(int id) {
-        uint32_t h = h1(id);
-        for (int p = 0; p < kVisitProbe; p++) {
-            int old = atomicCAS(&tab[(h + p) & mask], -1, id);
-            if (old == -1) return 0;
-            if (old == id) return 1;
-        }
-        unsigned long long key = ((unsigned long long)epoch << 32) | (uint32_t)id;
-        uint32_t g = h2(id);
-        uint32_t p = 0;
-        for (uint32_t it = 0; it < 4 * (gmask + 1) && p <= gmask; it++) {
-            unsigned long long* slot = &gtab[(g + p) & gmask];
-            unsigned long long v = *slot;
-            if ((uint32_t)(v >> 32) == epoch) {
-                if (v == key) return 1;
-                p++;
-                continue;
+    // Marks `id` (valid lanes only).  Returns a lane mask of ids that were not
+    // in the set before this call; an id repeated inside the call is credited
+    // to its first lane.  Sets fail when the table is full.
+    __device__ uint64_t mark(int id, bool valid, int lane, int& fail) {
+        tag = (tag + 1) & 31;
+        const uint32_t key = (tag << 27) | (uint32_t)(id + 1);
+        int r = 1;          // 0 inserted, 1 present, 2 present from this call, 3 full
+        uint32_t at = 0;
+        if (valid) {
+            uint32_t h = home(id);
+            r = 3;
+            for (uint32_t p = 0; p <= mask; p++) {
+                uint32_t s = (h + p) & mask;
+                uint32_t old = atomicCAS(&tab[s], 0u, key);
+                if (old == 0u) {
+                    r = 0;
+                    at = s;
+                    break;
+                }
+                if ((old & kVisitIdMask) == (uint32_t)(id + 1)) {
+                    r = (old >> 27) == tag ? 2 : 1;
+                    break;
+                }
             }
-            unsigned long long prev = atomicCAS(slot, v, key);
-            if (prev == v) return 0;
-            if (prev == key) return 1;
-            // another lane took the slot: look at it again
         }
-        return 2;
+        if (ballot(r == 3)) fail = 1;
+        uint64_t newm = ballot(r == 0);
+        uint64_t dupm = ballot(r == 2);
+        if (newm) {
+            if (r == 0) log[count + lanes_below(newm)] = (int32_t)at;
+            count += __builtin_popcountll(newm);
+        }
+        // same id twice in one chunk: the first lane is the new one (the
+        // reference scans the row in order, include/hnsw.hpp:413-422); a tag
+        // match left from 32 chunks ago finds no partner and changes nothing
+        while (dupm) {
+            int j = __builtin_ctzll(dupm);
+            int idj = readlane(id, j);
+            uint64_t m = ballot(valid && id == idj) & (newm | dupm);
+            uint64_t owner = m & newm;
+            if (owner) {
+                int first = __builtin_ctzll(m);
+                newm = (newm & ~m) | (1ull << first);
+            }
+            dupm &= ~m;
+        }
+        return newm;
+    }
+
+    // Empties the table (the slots this search filled) and orders the clears
+    // before the next search's atomics.
+    __device__ void clear(int lane) {
+        // log entries were written by other lanes: read them from L2
+        for (int i = lane; i < count; i += kWave) tab[__hip_atomic_load(&log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = 0u;
+        count = 0;
+        __builtin_amdgcn_s_waitcnt(0);  // stores reach L2 before the next atomics
     }
 };
 

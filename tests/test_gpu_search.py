@@ -73,11 +73,12 @@ def test_uneven_k_and_ef_per_query(graphs):
 
 
 def test_spill_paths_exact(graphs):
-    """Tiny LDS tables force the candidate heap and the visited set into
-    their device-memory spill; results stay identical."""
+    """A tiny LDS part forces the candidate heap into its device-memory
+    spill, and a small visited table forces long probe chains; results stay
+    identical."""
     fx, p, base, queries, g = graphs("f32_l2")
     idx = index_from_oracle(g)
-    idx.set_tuning(ccap=8, vbits=5)
+    idx.set_tuning(ccap=8, vbits=12)
     for k, ef in [(10, 80), (100, 128)]:
         tag = f"k{k}_ef{ef}"
         ids, ds, cnt = idx.search(queries, k, ef)

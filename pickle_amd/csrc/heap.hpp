@@ -19,7 +19,8 @@
 //             one is strictly better) is found in parallel, then the path
 //             from the root is followed with v_readlane until the moved
 //             entry no longer ranks below the child; the path shifts up.
-//   pop       the sequential sift-down (for heaps too large for pop_par).
+//   pop_deep  the sift-down three levels per round: lanes fetch the next
+//             three levels below the current node at once.
 #pragma once
 #include "wave.hpp"
 
@@ -87,37 +88,55 @@ struct WaveHeap {
         __builtin_amdgcn_wave_barrier();
     }
 
-    // sequential sift-down, for any size
-    __device__ void pop(bool leader) {
-        uint64_t x = get(len - 1);
-        len -= 1;
-        if (len == 0) return;
-        float xd = entry_dist(x);
-        int i = 0;
-        for (;;) {
-            int l = 2 * i + 1, r = l + 1;
-            if (l >= len) break;
-            int best = i;
-            uint64_t bv = x;
-            float bd = xd;
-            uint64_t le = get(l);
-            if (below(bd, entry_dist(le))) {
-                best = l;
-                bv = le;
-                bd = entry_dist(le);
-            }
-            if (r < len) {
-                uint64_t re = get(r);
-                if (below(bd, entry_dist(re))) {
-                    best = r;
-                    bv = re;
+    // Sift-down reading a three-level subtree per round.  Relative to the
+    // current node (position 1), lane t < 14 fetches subtree position t + 2
+    // (children 2-3, grandchildren 4-7, great-grandchildren 8-15); up to
+    // three levels are then resolved from those values with v_readlane.
+    __device__ void pop_deep(int lane, bool leader) {
+        const int n = uni(len) - 1;
+        const uint64_t x = get(n);
+        len = n;
+        if (n == 0) return;
+        const float xd = entry_dist(x);
+        const int p = lane + 2;
+        const int L = 31 - __builtin_clz((unsigned)p);
+        const int k = p - (1 << L);
+        int cur = 0;
+        bool going = true;
+        while (going) {
+            const int idx = ((cur + 1) << L) + k - 1;
+            const bool act = lane < 14 && idx < n;
+            const uint64_t v = act ? raw_get(idx) : 0;
+            const int vlo = (int)(uint32_t)v, vhi = (int)(uint32_t)(v >> 32);
+            __builtin_amdgcn_wave_barrier();
+            int rp = 1;
+            for (int lev = 0; lev < 3; lev++) {
+                const int li = 2 * cur + 1;
+                if (li >= n) {
+                    going = false;
+                    break;
                 }
+                const int lc = 2 * rp - 2;  // lane holding the left child
+                uint64_t bv = (uint64_t)(uint32_t)readlane(vlo, lc) | ((uint64_t)(uint32_t)readlane(vhi, lc) << 32);
+                int best = li;
+                if (li + 1 < n) {
+                    uint64_t rv = (uint64_t)(uint32_t)readlane(vlo, lc + 1) |
+                                  ((uint64_t)(uint32_t)readlane(vhi, lc + 1) << 32);
+                    if (below(entry_dist(bv), entry_dist(rv))) {
+                        best = li + 1;
+                        bv = rv;
+                    }
+                }
+                if (!below(xd, entry_dist(bv))) {
+                    going = false;
+                    break;
+                }
+                put(cur, bv, leader);
+                rp = 2 * rp + (best - li);
+                cur = best;
             }
-            if (best == i) break;
-            put(i, bv, leader);
-            i = best;
         }
-        put(i, x, leader);
+        put(cur, x, leader);
         __builtin_amdgcn_wave_barrier();
     }
 

@@ -9,10 +9,11 @@
 // counter until the batch is drained (uneven ef / k per query balance
 // themselves).  Per hop the wave
 //   1. pops the closest candidate (heap in LDS, deep levels in device memory),
-//   2. reads the adjacency row, one id per lane (coalesced 4-byte loads),
-//   3. tests and marks the ids in the exact visited set (one atomic per id),
-//   4. gathers the new rows (one lane group per row, 16-byte loads) and
-//      computes their distances,
+//   2. reads the degree and the adjacency row, one id per lane (coalesced
+//      4-byte loads, both in one round trip),
+//   3. tests and marks the ids in the exact visited set (one atomic per id)
+//      while it gathers every listed row (one lane group per row, 16-byte
+//      loads) and computes the distances, keeping those of new ids,
 //   5. applies the reference's accept rule in adjacency order, updating the
 //      result and candidate heaps exactly as the reference's binary heaps
 //      (each push and result-heap pop is one wave-parallel step).
@@ -74,12 +75,52 @@ __device__ void batch_distances(const DF& df, const uint8_t* vec, int vstride, c
     }
 }
 
-__device__ __forceinline__ int clamp_degree(const LayerDev& L, int v) {
-    int deg = L.deg[v];
+__device__ __forceinline__ int clamp_degree(const LayerDev& L, int v, int deg) {
     int64_t room = L.adj_len - (int64_t)v * L.D;
     if (deg < 0) deg = 0;
     if ((int64_t)deg > room) deg = (int)(room < 0 ? 0 : room);
     return deg;
+}
+
+// One hop's row: the degree and the first 64 adjacency slots are loaded
+// together (the adjacency allocation has 64 slots of padding past adj_len,
+// so the speculative read stays inside it).
+struct RowHead {
+    int64_t base;
+    int deg;
+    int first;  // this lane's id among the first 64 slots
+};
+
+__device__ __forceinline__ RowHead load_row_head(const LayerDev& L, int v, int lane) {
+    RowHead h;
+    h.base = (int64_t)v * L.D;
+    int d = L.deg[v];
+    h.first = L.adj[h.base + lane];
+    h.deg = uni(clamp_degree(L, v, d));
+    return h;
+}
+
+// Expands one chunk of 64 adjacency slots: the visited-set compare-and-swaps
+// and the gathers of every valid row go out in the same round trip (rows of
+// already-visited ids are fetched and their distances dropped).  Returns the
+// new lanes; this lane's id and (if new) distance come back through id / d.
+template <class DF>
+__device__ uint64_t expand_chunk(const DF& df, const LayerDev& L, int vstride, const RowHead& rh, int b0,
+                                 WaveVisited& vis, int* cl, float* cd, int lane, int& fail, int& id, float& d) {
+    const int j = b0 + lane;
+    const bool inrow = j < rh.deg;
+    id = b0 == 0 ? rh.first : (inrow ? L.adj[rh.base + j] : -1);
+    const bool valid = inrow && id >= 0 && (int64_t)id < L.n;
+    WaveVisited::Pending pend = vis.begin(id, valid);
+    const uint64_t vm = ballot(valid);
+    const int pos = lanes_below(vm);
+    if (valid) cl[pos] = id;
+    __builtin_amdgcn_wave_barrier();
+    batch_distances(df, L.vec, vstride, cl, cd, __builtin_popcountll(vm), lane);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t newm = vis.finish(pend, id, valid, fail);
+    d = ((newm >> lane) & 1) ? cd[pos] : __builtin_inff();
+    return newm;
 }
 
 template <class DF>
@@ -90,19 +131,6 @@ __device__ float one_distance(const DF& df, const LayerDev& L, int vstride, int 
     batch_distances(df, L.vec, vstride, cl, cd, 1, lane);
     __builtin_amdgcn_wave_barrier();
     return unif(cd[0]);
-}
-
-// distances for the lanes in `newm`; returns this lane's distance (+inf if not new)
-template <class DF>
-__device__ float chunk_distances(const DF& df, const LayerDev& L, int vstride, uint64_t newm, int id, int* cl,
-                                 float* cd, int lane) {
-    bool isnew = (newm >> lane) & 1;
-    int pos = lanes_below(newm);
-    if (isnew) cl[pos] = id;
-    __builtin_amdgcn_wave_barrier();
-    batch_distances(df, L.vec, vstride, cl, cd, __builtin_popcountll(newm), lane);
-    __builtin_amdgcn_wave_barrier();
-    return isnew ? cd[pos] : __builtin_inff();
 }
 
 template <class DF, bool IS_INT>
@@ -155,23 +183,19 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
         for (int level = A.enter_level; level >= 1 && !fail; level--) {
             const LayerDev L = A.layers[level];
             float cdist = one_distance(df, L, A.vstride, cur, cl, cd, lane);
-            vis.mark(cur, leader, lane, fail);
+            vis.mark(cur, leader, fail);
             bool updated;
             do {
                 updated = false;
-                const int scan = cur;
-                const int deg = uni(clamp_degree(L, scan));
+                const RowHead rh = load_row_head(L, cur, lane);
                 hopu++;
-                nbu += deg;
-                for (int b0 = 0; b0 < deg; b0 += kWave) {
-                    int j = b0 + lane;
-                    bool valid = j < deg;
-                    int id = valid ? L.adj[(int64_t)scan * L.D + j] : -1;
-                    valid = valid && id >= 0 && (int64_t)id < L.n;
-                    uint64_t newm = vis.mark(id, valid, lane, fail);
+                nbu += rh.deg;
+                for (int b0 = 0; b0 < rh.deg; b0 += kWave) {
+                    int id;
+                    float d;
+                    uint64_t newm = expand_chunk(df, L, A.vstride, rh, b0, vis, cl, cd, lane, fail, id, d);
                     if (newm == 0) continue;
                     dsu += __builtin_popcountll(newm);
-                    float d = chunk_distances(df, L, A.vstride, newm, id, cl, cd, lane);
                     float m;
                     int at = wave_min_lane(d, (newm >> lane) & 1, &m);
                     if (at >= 0 && m < cdist) {  // strictly closer, first in row order
@@ -193,25 +217,21 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
             float d0 = one_distance(df, L, A.vstride, cur, cl, cd, lane);
             W.push_par(pack_entry(d0, cur), lane, leader);
             C.push_par(pack_entry(d0, cur), lane, leader);
-            vis.mark(cur, leader, lane, fail);
+            vis.mark(cur, leader, fail);
             uint32_t wtop_bits = __float_as_uint(d0);  // bits of W.top()
             while (C.len > 0 && !fail) {
                 const uint64_t te = C.top();
-                C.pop(leader);
+                C.pop_deep(lane, leader);
                 if (entry_dist(te) > __uint_as_float(wtop_bits)) break;  // hnsw.hpp:403
-                const int c = entry_id(te);
-                const int deg = uni(clamp_degree(L, c));
+                const RowHead rh = load_row_head(L, entry_id(te), lane);
                 hop0++;
-                nb0 += deg;
-                for (int b0 = 0; b0 < deg && !fail; b0 += kWave) {
-                    int j = b0 + lane;
-                    bool valid = j < deg;
-                    int id = valid ? L.adj[(int64_t)c * L.D + j] : -1;
-                    valid = valid && id >= 0 && (int64_t)id < L.n;
-                    uint64_t newm = vis.mark(id, valid, lane, fail);
+                nb0 += rh.deg;
+                for (int b0 = 0; b0 < rh.deg && !fail; b0 += kWave) {
+                    int id;
+                    float d;
+                    uint64_t newm = expand_chunk(df, L, A.vstride, rh, b0, vis, cl, cd, lane, fail, id, d);
                     if (newm == 0) continue;
                     ds0 += __builtin_popcountll(newm);
-                    float d = chunk_distances(df, L, A.vstride, newm, id, cl, cd, lane);
                     bool isnew = (newm >> lane) & 1;
                     const float wtop = __uint_as_float(wtop_bits);
                     uint64_t cand = W.len >= ef ? ballot(isnew && d < wtop) : newm;

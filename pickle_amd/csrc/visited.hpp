@@ -32,20 +32,38 @@ struct WaveVisited {
         return ((uint32_t)id * 0x9E3779B1u) >> (32 - bits);
     }
 
-    // Marks `id` (valid lanes only).  Returns a lane mask of ids that were not
-    // in the set before this call; an id repeated inside the call is credited
-    // to its first lane.  Sets fail when the table is full.
-    __device__ uint64_t mark(int id, bool valid, int lane, int& fail) {
+    // Marks `id` (valid lanes only), in two halves so the caller can put
+    // other memory work between them: begin() issues the first
+    // compare-and-swap, finish() resolves it (and any collision chain).
+    // finish() returns a lane mask of ids that were not in the set before;
+    // an id repeated inside the call is credited to its first lane.  Sets
+    // fail when the table is full.
+    struct Pending {
+        uint32_t key;
+        uint32_t slot;
+        uint32_t old;
+    };
+
+    __device__ __forceinline__ Pending begin(int id, bool valid) {
         tag = (tag + 1) & 31;
-        const uint32_t key = (tag << 27) | (uint32_t)(id + 1);
-        int r = 1;          // 0 inserted, 1 present, 2 present from this call, 3 full
+        Pending p;
+        p.key = (tag << 27) | (uint32_t)(id + 1);
+        p.slot = valid ? home(id) : 0u;
+        p.old = valid ? atomicCAS(&tab[p.slot], 0u, p.key) : 0xffffffffu;
+        return p;
+    }
+
+    __device__ uint64_t finish(const Pending& pd, int id, bool valid, int& fail) {
+        int r = 1;  // 0 inserted, 1 present, 2 present from this call, 3 full
         uint32_t at = 0;
         if (valid) {
-            uint32_t h = home(id);
+            uint32_t s = pd.slot, old = pd.old;
             r = 3;
             for (uint32_t p = 0; p <= mask; p++) {
-                uint32_t s = (h + p) & mask;
-                uint32_t old = atomicCAS(&tab[s], 0u, key);
+                if (p > 0) {
+                    s = (pd.slot + p) & mask;
+                    old = atomicCAS(&tab[s], 0u, pd.key);
+                }
                 if (old == 0u) {
                     r = 0;
                     at = s;
@@ -79,6 +97,11 @@ struct WaveVisited {
             dupm &= ~m;
         }
         return newm;
+    }
+
+    __device__ __forceinline__ uint64_t mark(int id, bool valid, int& fail) {
+        Pending p = begin(id, valid);
+        return finish(p, id, valid, fail);
     }
 
     // Empties the table (the slots this search filled) and orders the clears

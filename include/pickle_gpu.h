@@ -170,6 +170,30 @@ int pk_brute_force(pk_index *h, const void *queries, int64_t nq, int32_t *out_id
 /* Frequency of the device wall clock behind latency_ticks, in kHz. */
 int pk_wall_clock_khz(int *khz);
 
+/* ---- insertion ---- */
+
+/* pk_index_insert flag: reproduce the reference's AddEdge exactly, including
+ * its write one slot past a full row (include/hnsw_layer.hpp:441-447); with
+ * max_batch = 1 the build equals the reference's single-threaded Build.
+ * Without it rows never exceed D entries. */
+#define PK_BUILD_EXACT_ROWS 1
+
+/* Level sampler of include/util.hpp:13-64: a fresh uniform per level,
+ * P(level >= l) = M^(-l(l+1)/2); seeded (the reference seeds from
+ * random_device). */
+int pk_sample_levels(int64_t n, int M, uint64_t seed, uint8_t *out);
+
+/* InitBuffer (include/hnsw.hpp:222-245): levels[capacity] is every external
+ * id's insert level; sizes every layer for them and empties the graph.  Base
+ * rows must be set. */
+int pk_index_init_build(pk_index *h, const uint8_t *levels);
+
+/* Insert (include/hnsw.hpp:247-302) for ext_ids in order, on the GPU in
+ * batches of up to max_batch nodes (batches grow with the graph; a node that
+ * opens a new top level goes alone).  Build (:497-507) is init_build + one
+ * insert of all ids.  flags: PK_BUILD_EXACT_ROWS or 0.  Synchronous. */
+int pk_index_insert(pk_index *h, const int32_t *ext_ids, int64_t n, int max_batch, int flags);
+
 /* Waits for all work enqueued on the handle's stream. */
 int pk_sync(pk_index *h);
 

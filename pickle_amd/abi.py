@@ -28,8 +28,10 @@ EXPORTS = [
     "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
-    "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz",
+    "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
+    "pk_index_insert",
 ]
+BUILD_EXACT_ROWS = 1
 BRUTE_K = 32
 
 
@@ -79,6 +81,9 @@ def lib():
             "pk_timer_end": (C.c_int, [vp, C.POINTER(C.c_float)]),
             "pk_brute_force": (C.c_int, [vp, vp, i64, vp, vp]),
             "pk_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
+            "pk_sample_levels": (C.c_int, [i64, C.c_int, C.c_uint64, vp]),
+            "pk_index_init_build": (C.c_int, [vp, vp]),
+            "pk_index_insert": (C.c_int, [vp, vp, i64, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -97,6 +102,13 @@ def wall_clock_khz() -> int:
     k = C.c_int()
     check(lib().pk_wall_clock_khz(C.byref(k)))
     return k.value
+
+
+def sample_levels(n: int, M: int, seed: int) -> np.ndarray:
+    """Insert levels with the reference's distribution (include/util.hpp:38-64)."""
+    out = np.zeros(n, np.uint8)
+    check(lib().pk_sample_levels(n, M, seed, out.ctypes.data_as(C.c_void_p)))
+    return out
 
 
 def _p(a):
@@ -256,6 +268,29 @@ class Index:
         check(lib().pk_search_device_ex(self.h, d_queries.ptr, nq, d_k.ptr if d_k else None, d_ef.ptr if d_ef else None,
                                         k, ef, ef_max if ef_max else ef, k_stride, d_ids.ptr, d_dists.ptr,
                                         d_count.ptr if d_count else None, d_ctr.ptr if d_ctr else None))
+
+    def init_build(self, levels: np.ndarray) -> None:
+        """InitBuffer: levels for every external id of the capacity."""
+        lv = np.ascontiguousarray(levels, np.uint8)
+        if lv.shape[0] != self.capacity:
+            raise ValueError("levels must cover the whole capacity")
+        check(lib().pk_index_init_build(self.h, _p(lv)))
+
+    def insert(self, ext_ids, max_batch: int = 4096, exact_rows: bool = False) -> None:
+        ids = np.ascontiguousarray(ext_ids, np.int32)
+        check(lib().pk_index_insert(self.h, _p(ids), ids.shape[0], max_batch,
+                                    BUILD_EXACT_ROWS if exact_rows else 0))
+
+    def build(self, base: np.ndarray, levels: np.ndarray | None = None, order=None, max_batch: int = 4096,
+              exact_rows: bool = False, seed: int = 0) -> None:
+        """Build (include/hnsw.hpp:497-507): base rows, levels, inserts."""
+        self.set_base(base)
+        if levels is None:
+            levels = sample_levels(self.capacity, self.M, seed)
+        self.init_build(levels)
+        if order is None:
+            order = np.arange(base.shape[0], dtype=np.int32)
+        self.insert(order, max_batch=max_batch, exact_rows=exact_rows)
 
     def brute_force(self, queries: np.ndarray):
         """The BRUTE_K nearest base rows per query (exact for u8/i8; FP32 for floats)."""

@@ -86,6 +86,7 @@ void dfree(T*& p) {
 }
 
 void free_layer(Layer& L) {
+    dfree(L.lock);
     dfree(L.adj);
     dfree(L.deg);
     dfree(L.heur);
@@ -103,6 +104,95 @@ namespace pk {
 int set_error(int code, const std::string& msg) {
     g_err = msg;
     return code;
+}
+
+void free_layer_public(Layer& L) { free_layer(L); }
+
+// Uploads the layer descriptors the kernels read (sizes, pointers).  Waits
+// for the index stream first, so no running kernel sees a half-updated table.
+int publish_layers(pk_index* h) {
+    struct LayerBuildDesc {
+        int32_t* adj;
+        int32_t* deg;
+        int16_t* heur;
+        float* dist;
+        int32_t* ext;
+        uint8_t* vec;
+        uint8_t* has;
+        int32_t* down;
+        int32_t* lock;
+        int64_t adj_len;
+        int32_t D;
+        int32_t pad_;
+    };
+    LayerDev desc[PK_MAX_LEVELS] = {};
+    LayerBuildDesc bdesc[PK_MAX_LEVELS] = {};
+    for (int l = 0; l < PK_MAX_LEVELS; l++) {
+        const Layer& L = h->layers[l];
+        if (!L.loaded) continue;
+        desc[l] = LayerDev{L.adj, L.deg, L.vec, L.down, L.n, L.adj_len, L.D, 0};
+        bdesc[l] = LayerBuildDesc{L.adj, L.deg, L.heur, L.dist, L.ext, L.vec, L.has, L.down, L.lock, L.adj_len, L.D, 0};
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!h->d_layers) HIPCHK(hipMalloc(&h->d_layers, sizeof(desc)));
+    if (!h->d_build) HIPCHK(hipMalloc(&h->d_build, sizeof(bdesc)));
+    HIPCHK(hipMemcpy(h->d_layers, desc, sizeof(desc), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_build, bdesc, sizeof(bdesc), hipMemcpyHostToDevice));
+    return PK_OK;
+}
+
+int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits) {
+    InsertScratch& s = h->ins;
+    if (nodes > s.cap_nodes || maxl > s.cap_maxl || topk > s.topk) {
+        dfree(s.d_ext);
+        dfree(s.d_lvl);
+        dfree(s.d_in);
+        dfree(s.r_ids);
+        dfree(s.r_d);
+        dfree(s.r_cnt);
+        int rc;
+        size_t lists = (size_t)nodes * maxl * 2;
+        if ((rc = dalloc(&s.d_ext, nodes)) || (rc = dalloc(&s.d_lvl, nodes)) ||
+            (rc = dalloc(&s.d_in, (size_t)nodes * maxl)) || (rc = dalloc(&s.r_ids, lists * topk)) ||
+            (rc = dalloc(&s.r_d, lists * topk)) || (rc = dalloc(&s.r_cnt, lists)))
+            return rc;
+        s.cap_nodes = nodes;
+        s.cap_maxl = maxl;
+        s.topk = topk;
+    }
+    if (waves > s.waves || vbits > s.vbits) {
+        dfree(s.cglob);
+        dfree(s.vtab);
+        dfree(s.vlog);
+        int rc;
+        size_t slots = (size_t)waves << vbits;
+        if ((rc = dalloc(&s.cglob, (size_t)waves * h->cg_cap)) || (rc = dalloc(&s.vtab, slots)) ||
+            (rc = dalloc(&s.vlog, slots)))
+            return rc;
+        HIPCHK(hipMemset(s.vtab, 0, slots * sizeof(uint32_t)));
+        s.waves = waves;
+        s.vbits = vbits;
+    }
+    if (!s.counter) {
+        int rc;
+        if ((rc = dalloc(&s.counter, 1)) || (rc = dalloc(&s.err, 1))) return rc;
+        HIPCHK(hipMemset(s.err, 0, 4));
+    }
+    return PK_OK;
+}
+
+int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
+                        int maxl) {
+    InsertScratch& s = h->ins;
+    if (nb > s.cap_nodes || maxl > s.cap_maxl) {
+        int rc = ensure_insert_scratch(h, std::max(nb, s.cap_nodes), std::max(maxl, s.cap_maxl), s.topk, s.waves,
+                                       s.vbits);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpy(s.d_ext, ext, nb * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.d_lvl, lvl, nb * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.d_in, in, (size_t)nb * maxl * 4, hipMemcpyHostToDevice));
+    return PK_OK;
 }
 }  // namespace pk
 
@@ -157,6 +247,18 @@ int pk_index_destroy(pk_index* h) {
     dfree(h->vtab);
     dfree(h->vlog);
     if (h->d_layers) (void)hipFree(h->d_layers);
+    if (h->d_build) (void)hipFree(h->d_build);
+    dfree(h->ins.d_ext);
+    dfree(h->ins.d_lvl);
+    dfree(h->ins.d_in);
+    dfree(h->ins.r_ids);
+    dfree(h->ins.r_d);
+    dfree(h->ins.r_cnt);
+    dfree(h->ins.cglob);
+    dfree(h->ins.vtab);
+    dfree(h->ins.vlog);
+    dfree(h->ins.counter);
+    dfree(h->ins.err);
     dfree(h->qcounter);
     dfree(h->err);
     for (auto& b : h->io) dfree(b.p);
@@ -195,8 +297,11 @@ int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_le
     Layer& L = h->layers[level];
     free_layer(L);
     L.n = n;
+    L.cap = n;
     L.D = D;
     L.adj_len = adj_len;
+    L.built = false;
+    h->building = false;
     const int64_t pad = 64;  // a chunk may read up to one wave past the degree
     int rc;
     if ((rc = dalloc(&L.adj, (size_t)(adj_len + pad))) || (rc = dalloc(&L.deg, (size_t)n)) ||
@@ -240,6 +345,7 @@ int pk_index_set_enter_level(pk_index* h, int enter_level) {
     for (int l = 1; l <= enter_level; l++) {
         Layer& lo = h->layers[l - 1];
         Layer& up = h->layers[l];
+        if (up.built) continue;  // the insertion kernels write these
         std::fill(ext2in.begin(), ext2in.end(), -1);
         for (int64_t i = 0; i < lo.n; i++) {
             int32_t e = lo.h_ext[i];
@@ -255,13 +361,8 @@ int pk_index_set_enter_level(pk_index* h, int enter_level) {
         }
         HIPCHK(hipMemcpy(up.down, down.data(), up.n * sizeof(int32_t), hipMemcpyHostToDevice));
     }
-    LayerDev desc[PK_MAX_LEVELS] = {};
-    for (int l = 0; l <= enter_level; l++) {
-        const Layer& L = h->layers[l];
-        desc[l] = LayerDev{L.adj, L.deg, L.vec, L.down, L.n, L.adj_len, L.D, 0};
-    }
-    if (!h->d_layers) HIPCHK(hipMalloc(&h->d_layers, sizeof(desc)));
-    HIPCHK(hipMemcpy(h->d_layers, desc, sizeof(desc), hipMemcpyHostToDevice));
+    int rc = publish_layers(h);
+    if (rc) return rc;
     h->enter_level = enter_level;
     h->ready = true;
     return PK_OK;

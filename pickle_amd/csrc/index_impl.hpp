@@ -11,7 +11,9 @@ namespace pk {
 
 struct Layer {
     bool loaded = false;
-    int64_t n = 0;
+    bool built = false;  // created by pk_index_init_build (rows written by the GPU)
+    int64_t n = 0;       // nodes present (NewInID's next id)
+    int64_t cap = 0;     // rows allocated
     int32_t D = 0;
     int64_t adj_len = 0;
     int32_t* adj = nullptr;
@@ -22,7 +24,28 @@ struct Layer {
     uint8_t* vec = nullptr;
     int32_t* down = nullptr;
     uint8_t* has = nullptr;
-    std::vector<int32_t> h_ext;  // host copy, for the down maps
+    int32_t* lock = nullptr;     // per-node AddEdge locks (built layers)
+    std::vector<int32_t> h_ext;  // host copy, for the down maps of loaded layers
+};
+
+// device scratch of the insertion kernels
+struct InsertScratch {
+    int32_t* d_ext = nullptr;
+    int32_t* d_lvl = nullptr;
+    int32_t* d_in = nullptr;
+    int64_t cap_nodes = 0;
+    int cap_maxl = 0;
+    int32_t* r_ids = nullptr;
+    float* r_d = nullptr;
+    int32_t* r_cnt = nullptr;
+    int topk = 0;
+    uint64_t* cglob = nullptr;
+    uint32_t* vtab = nullptr;
+    int32_t* vlog = nullptr;
+    int waves = 0;
+    int vbits = 0;
+    int32_t* counter = nullptr;
+    int32_t* err = nullptr;
 };
 
 struct SearchTuning {
@@ -41,6 +64,16 @@ int set_error(int code, const std::string& msg);
 
 }  // namespace pk
 
+struct pk_index;
+
+namespace pk {
+void free_layer_public(Layer& L);
+int publish_layers(pk_index* h);
+int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits);
+int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
+                        int maxl);
+}  // namespace pk
+
 struct pk_index {
     int dtype = 0, metric = 0, dim = 0, M = 0, efc = 0;
     int64_t capacity = 0;
@@ -54,6 +87,10 @@ struct pk_index {
     int64_t base_rows = 0;
     pk::Layer layers[PK_MAX_LEVELS];
     void* d_layers = nullptr;  // device copy of the kernels' layer descriptors
+    void* d_build = nullptr;   // device copy of the insertion kernels' layer descriptors
+    std::vector<uint8_t> levels;  // insert level per external id (InitBuffer)
+    bool building = false;
+    pk::InsertScratch ins;
     int enter_level = 0;
     bool ready = false;
     // search scratch (per resident wave)

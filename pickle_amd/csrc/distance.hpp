@@ -47,21 +47,20 @@ struct IntDist {
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
             int c = g + 8 * i;
-            q[i] = c < nchunks ? *reinterpret_cast<const uint4*>(qst + 16 * c) : make_uint4(0, 0, 0, 0);
+            q[i] = c < nchunks ? ld16_lds(qst + 16 * c) : make_uint4(0, 0, 0, 0);
             part = dot4(SIGNED, q[i].x, q[i].x, part);
             part = dot4(SIGNED, q[i].y, q[i].y, part);
             part = dot4(SIGNED, q[i].z, q[i].z, part);
             part = dot4(SIGNED, q[i].w, q[i].w, part);
         }
-        qq = group_sum<8>((int)part);
+        qq = sum8((int)part);
     }
 
     __device__ void load_row(Row& r, const uint8_t* row, int g, bool active) const {
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
             int c = g + 8 * i;
-            r.c[i] = (active && c < nchunks) ? reinterpret_cast<const uint4*>(row)[c]
-                                             : make_uint4(0, 0, 0, 0);
+            r.c[i] = (active && c < nchunks) ? ld16_global(row + 16 * c) : make_uint4(0, 0, 0, 0);
         }
     }
 
@@ -81,19 +80,25 @@ struct IntDist {
                 bb = dot4(SIGNED, r.c[i].w, r.c[i].w, bb);
             }
         }
-        int sab = group_sum<8>((int)ab);
+        int sab = sum8((int)ab);
         if (MET == MET_L2) {
-            int sbb = group_sum<8>((int)bb);
+            int sbb = sum8((int)bb);
             return (float)(qq + sbb - 2 * sab);
         }
         return -(float)sab;
     }
 };
 
+// element i of a row in device memory / of the query staged in LDS
 template <bool HALF>
 __device__ __forceinline__ float felem(const uint8_t* p, int i) {
-    if (HALF) return (float)reinterpret_cast<const _Float16*>(p)[i];
-    return reinterpret_cast<const float*>(p)[i];
+    if (HALF) return (float)gbl(reinterpret_cast<const _Float16*>(p))[i];
+    return gbl(reinterpret_cast<const float*>(p))[i];
+}
+template <bool HALF>
+__device__ __forceinline__ float qelem(const uint8_t* p, int i) {
+    if (HALF) return (float)lds_ptr(reinterpret_cast<const _Float16*>(p))[i];
+    return lds_ptr(reinterpret_cast<const float*>(p))[i];
 }
 
 template <int MET>
@@ -123,7 +128,7 @@ struct FloatDist {
         nmain = dim >> 4;
         rem = dim & 15;
 #pragma unroll
-        for (int i = 0; i < NEL; i++) q[i] = i < nmain ? felem<HALF>(qst, 16 * i + g) : 0.0f;
+        for (int i = 0; i < NEL; i++) q[i] = i < nmain ? qelem<HALF>(qst, 16 * i + g) : 0.0f;
     }
 
     __device__ void load_row(Row& r, const uint8_t* row, int g, bool active) const {
@@ -138,7 +143,7 @@ struct FloatDist {
         if (rem >= 8) {
             float a[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) a[u] = fterm<MET>(felem<HALF>(qst, base + u), felem<HALF>(row, base + u), 0.0f);
+            for (int u = 0; u < 8; u++) a[u] = fterm<MET>(qelem<HALF>(qst, base + u), felem<HALF>(row, base + u), 0.0f);
 #pragma unroll
             for (int h = 4; h >= 1; h >>= 1)
 #pragma unroll
@@ -150,13 +155,13 @@ struct FloatDist {
             float a[4];
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                a[u] = fterm<MET>(felem<HALF>(qst, base + j + u), felem<HALF>(row, base + j + u), u == 0 ? r : 0.0f);
+                a[u] = fterm<MET>(qelem<HALF>(qst, base + j + u), felem<HALF>(row, base + j + u), u == 0 ? r : 0.0f);
             a[0] = a[0] + a[2];
             a[1] = a[1] + a[3];
             r = a[0] + a[1];
             j += 4;
         }
-        for (; j < rem; j++) r = fterm<MET>(felem<HALF>(qst, base + j), felem<HALF>(row, base + j), r);
+        for (; j < rem; j++) r = fterm<MET>(qelem<HALF>(qst, base + j), felem<HALF>(row, base + j), r);
         return r;
     }
 

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
         for (int level = A.enter_level; level >= 1 && !fail; level--) {
             const LayerDev L = A.layers[level];
             cur = greedy_level(df, L, A.vstride, cur, vis, cl, cd, lane, fail, up);
-            if (!fail) cur = L.down[cur];
+            if (!fail) cur = gbl(L.down)[cur];
         }
 
         // ---- layer 0: SearchLayerFast (include/hnsw.hpp:374-444) ----

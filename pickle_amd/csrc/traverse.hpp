@@ -81,8 +81,8 @@ struct RowHead {
 __device__ __forceinline__ RowHead load_row_head(const LayerDev& L, int v, int lane) {
     RowHead h;
     h.base = (int64_t)v * L.D;
-    int d = L.deg[v];
-    h.first = L.adj[h.base + lane];
+    int d = gbl(L.deg)[v];
+    h.first = gbl(L.adj)[h.base + lane];
     h.deg = uni(clamp_degree(L, v, d));
     return h;
 }
@@ -96,7 +96,7 @@ __device__ uint64_t expand_chunk(const DF& df, const LayerDev& L, int vstride, c
                                  WaveVisited& vis, int* cl, float* cd, int lane, int& fail, int& id, float& d) {
     const int j = b0 + lane;
     const bool inrow = j < rh.deg;
-    id = b0 == 0 ? rh.first : (inrow ? L.adj[rh.base + j] : -1);
+    id = b0 == 0 ? rh.first : (inrow ? gbl(L.adj)[rh.base + j] : -1);
     const bool valid = inrow && id >= 0 && (int64_t)id < L.n;
     WaveVisited::Pending pend = vis.begin(id, valid);
     const uint64_t vm = ballot(valid);

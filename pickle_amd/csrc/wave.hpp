@@ -35,22 +35,76 @@ __device__ __forceinline__ float readlanef(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// butterfly sums inside aligned groups of 2^k lanes (every lane gets the sum)
-template <int GROUP>
-__device__ __forceinline__ int group_sum(int v) {
-#pragma unroll
-    for (int m = GROUP / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+// Device-memory views.  Pointers read from memory (the layer table) are
+// generic; loads through them compile to flat instructions, which count
+// against the LDS wait counter as well, so every LDS wait would also wait
+// for the wave's outstanding HBM gathers.  Casting to the global address
+// space gives global_load instructions that only the vector-memory counter
+// tracks.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ const gmem<T>* gbl(const T* p) {
+    return (const gmem<T>*)p;
+}
+template <class T>
+__device__ __forceinline__ gmem<T>* gbl_rw(T* p) {
+    return (gmem<T>*)p;
+}
+template <class T>
+using lmem = __attribute__((address_space(3))) T;
+template <class T>
+__device__ __forceinline__ const lmem<T>* lds_ptr(const T* p) {
+    return (const lmem<T>*)p;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// 16-byte loads from device memory / LDS
+__device__ __forceinline__ uint4 ld16_global(const void* p) {
+    u32x4 v = *(const gmem<u32x4>*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ld16_lds(const void* p) {
+    u32x4 v = *(const lmem<u32x4>*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Cross-lane moves inside rows of 16 lanes by DPP (VALU only, no LDS
+// crossbar).  Controls: quad_perm 0x00-0xff, row_ror:n 0x120+n,
+// row_half_mirror 0x141.
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(dpp<CTRL>(__float_as_int(v)));
+}
+constexpr int kDppXor1 = 0xb1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4e;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <- lane 7-i within each 8
+template <int N>
+constexpr int kDppRor = 0x120 + N;    // row_ror:N
+
+// Sums inside aligned groups of 8 lanes (every lane gets the sum): pairs,
+// quads, then the two quads of the group meet through the half-row mirror.
+__device__ __forceinline__ int sum8(int v) {
+    v += dpp<kDppXor1>(v);
+    v += dpp<kDppXor2>(v);
+    v += dpp<kDppHalfMirror>(v);
     return v;
 }
 
 // Halving-tree float sum over aligned groups of 16 lanes: lanes j and j+8,
-// then j and j+4, j+2, j+1 (the xor partner holds the same value after each
-// step, and IEEE addition commutes, so every lane ends with the tree sum).
+// then j and j+4, j+2, j+1.  Rotating the row by 8, 4, 2, 1 pairs every
+// lane with the same partner set; each partial sum is an addition of two
+// values that are bit-identical to the tree's (IEEE addition commutes), so
+// every lane ends with the tree sum.
 __device__ __forceinline__ float tree_sum16(float v) {
-    v = v + __shfl_xor(v, 8, kWave);
-    v = v + __shfl_xor(v, 4, kWave);
-    v = v + __shfl_xor(v, 2, kWave);
-    v = v + __shfl_xor(v, 1, kWave);
+    v = v + dppf<kDppRor<8>>(v);
+    v = v + dppf<kDppRor<4>>(v);
+    v = v + dppf<kDppRor<2>>(v);
+    v = v + dppf<kDppRor<1>>(v);
     return v;
 }
 

@@ -37,6 +37,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS at recall@10>=0.95 (1 GPU vs host CPU); p99 latency; achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+CALIB_BYTES = 256 << 20
+PROFILES = os.path.join(ROOT, "profiles")
 
 WORKLOADS = {
     "gm1m_u8": dict(desc="1M x 128 uint8, 64-cluster gaussian mixture quantised to [0,255] (seed 1), L2, "
@@ -117,6 +119,117 @@ def read_ref_results(path, nq, k):
 
 
 # --------------------------------------------------------------------------
+# rocprofv3 summaries (profiles/) and the traffic they record
+# --------------------------------------------------------------------------
+
+def pmc_profile_path(workload, ef):
+    return os.path.join(PROFILES, f"k_search_{workload}_ef{ef}_pmc.md")
+
+
+def profiled_traffic(workload, n, nq, k, ef):
+    """HBM bytes per k_search launch recorded by a --summarize-pmc run of the
+    same workload, size and ef (None if there is none)."""
+    path = pmc_profile_path(workload, ef)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic-json: "):
+            rec = json.loads(line[len("traffic-json: "):])
+            if (rec["n"], rec["nq"], rec["k"], rec["ef"]) == (n, nq, k, ef):
+                return rec["traffic_bytes"], os.path.relpath(path, ROOT)
+    return None, None
+
+
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    rows = []
+    for f in files:
+        rows.extend(csv.DictReader(open(f)))
+    return rows
+
+
+def _short_kernel(name):
+    for key in ("k_search", "k_brute", "k_copy16", "k_gather_rows", "k_fill_i32", "k_insert_search",
+                "k_insert_rows", "k_insert_edges", "k_rerank"):
+        if key in name:
+            return key + (name[name.index("<"):name.index(">") + 1] if "<" in name and key != "k_copy16" else "")
+    return name[:60]
+
+
+def _counter_per_dispatch(directory, counter):
+    out = {}
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        if r["Counter_Name"] != counter:
+            continue
+        out.setdefault(_short_kernel(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize(args):
+    """Writes a profiles/ summary from rocprofv3 output and the bench line of
+    the same command."""
+    bl = json.loads(open(args.bench_json).read().strip().splitlines()[-1])
+    cfg, roof = bl["config"], bl["roofline"]
+    head = [f"# k_search — {cfg['workload']}", "",
+            f"ef={cfg['ef']}, k={cfg['k']}, {cfg['nq']} queries per launch, recall@10 = {cfg['recall@10']}", "",
+            "Command (GPU box, committed files only):", ""]
+    head += ["    " + c for c in args.cmd]
+    body = []
+    if args.summarize_trace:
+        rows = _csv_rows(args.summarize_trace, "kernel_stats.csv")
+        body += ["", "`kernel_stats.csv` (per-kernel totals over the whole process):", "",
+                 "| Kernel | Calls | Average | Min | Max |", "|---|---|---|---|---|"]
+        search_avg = None
+        for r in rows:
+            avg = float(r["AverageNs"]) / 1e6
+            name = _short_kernel(r["Name"])
+            if name.startswith("k_search"):
+                search_avg = avg
+            body.append(f"| `{name}` | {r['Calls']} | {avg:.4f} ms | {float(r['MinNs']) / 1e6:.4f} ms | "
+                        f"{float(r['MaxNs']) / 1e6:.4f} ms |")
+        body += ["", f"bench.py in the same run: {roof['kernel_ms']} ms per k_search launch (HIP events on the "
+                 f"index stream; trace average {search_avg:.4f} ms), {bl['value']:,.0f} QPS, algorithmic "
+                 f"{roof['algorithmic_bytes_per_launch'] / 1e9:.3f} GB per launch -> {roof['achieved']} GB/s "
+                 f"= {roof['frac'] * 100:.2f} % of 8 TB/s.",
+                 f"Per query: {roof['per_query']['hops']} hops, {roof['per_query']['neighbors']} adjacency "
+                 f"entries scanned, {roof['per_query']['dists']} distances."]
+    if args.summarize_pmc:
+        fdir, wdir = args.summarize_pmc
+        fetch = _counter_per_dispatch(fdir, "FETCH_SIZE")
+        write = _counter_per_dispatch(wdir, "WRITE_SIZE")
+        kib = CALIB_BYTES / 1024.0
+        ff = (sum(fetch["k_copy16"]) / len(fetch["k_copy16"])) / kib
+        fw = (sum(write["k_copy16"]) / len(write["k_copy16"])) / kib
+        skey = [key for key in fetch if key.startswith("k_search")][0]
+        f_kib = sum(fetch[skey]) / len(fetch[skey])
+        w_kib = sum(write[skey]) / len(write[skey])
+        fb, wb = f_kib * 1024 / ff, w_kib * 1024 / fw
+        alg = roof["algorithmic_bytes_per_launch"]
+        body += ["", "Hardware byte counters (two `--pmc` passes, KiB per dispatch, averaged over the "
+                 "dispatches of each kernel):", "",
+                 "| Kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |", "|---|---|---|---|"]
+        for key in sorted(set(fetch) | set(write)):
+            fv, wv = fetch.get(key, [0]), write.get(key, [0])
+            body.append(f"| `{key}` | {len(fv)} | {sum(fv) / len(fv):,.1f} | {sum(wv) / len(wv):,.1f} |")
+        body += ["", f"Calibration: `k_copy16` reads and writes {CALIB_BYTES >> 20} MiB "
+                 f"({kib:,.0f} KiB); FETCH_SIZE reads {ff:.4f} x and WRITE_SIZE {fw:.4f} x that.",
+                 f"k_search per launch: {fb / 1e9:.3f} GB fetched + {wb / 1e9:.3f} GB written = "
+                 f"{(fb + wb) / 1e9:.3f} GB of HBM traffic, against {alg / 1e9:.3f} GB algorithmic "
+                 f"({(fb + wb) / alg:.2f} x).", ""]
+        rec = {"workload": args.workload, "n": cfg["n"], "nq": cfg["nq"], "k": cfg["k"], "ef": cfg["ef"],
+               "fetch_bytes": int(fb), "write_bytes": int(wb), "traffic_bytes": int(fb + wb),
+               "algorithmic_bytes": alg, "factor_fetch": round(ff, 5), "factor_write": round(fw, 5)}
+        body.append("traffic-json: " + json.dumps(rec))
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as f:
+        f.write("\n".join(head + body) + "\n")
+    print(f"wrote {args.out}")
+
 
 def recall_at_k(ids, gt, k):
     """benchmark/hnsw_pickle.cpp:94-112: a returned id counts if it is one of
@@ -142,7 +255,18 @@ def main():
     ap.add_argument("--graph-cache", default=None, help="directory to reuse / keep the CPU-built graph")
     ap.add_argument("--no-cpu-search", action="store_true", help="skip the CPU search baseline")
     ap.add_argument("--cpu-repeat", type=int, default=3)
+    ap.add_argument("--summarize-trace", default=None, metavar="DIR",
+                    help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")
+    ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write a profiles/ summary from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE directories")
+    ap.add_argument("--bench-json", default=None, help="the JSON line bench.py printed in the profiled run")
+    ap.add_argument("--out", default=None, help="summary file to write")
+    ap.add_argument("--cmd", action="append", default=[], help="command lines that produced the profile")
     args = ap.parse_args()
+
+    if args.summarize_trace or args.summarize_pmc:
+        summarize(args)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (run with --gpus 1). "
@@ -272,6 +396,16 @@ def main():
     log(f"timed: {args.steps} x {nq} queries at ef={chosen}: {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
 
+    # known-size copy (k_copy16, 256 MiB read + 256 MiB written, twice):
+    # the kernel against which profiled HBM byte counters are calibrated
+    calib = abi.DeviceBuffer(CALIB_BYTES)
+    calib2 = abi.DeviceBuffer(CALIB_BYTES)
+    for _ in range(2):
+        idx.device_copy(calib2, calib, CALIB_BYTES)
+    idx.sync()
+    calib.free()
+    calib2.free()
+
     # cpu_baseline leg: the reference's AnnSearch on the host cores
     cpu = None
     if not args.no_cpu_search:
@@ -290,6 +424,8 @@ def main():
         log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads, recall {cpu_recall:.4f}, "
             f"GPU ids identical on {same * 100:.2f}% of queries")
 
+    traffic, traffic_src = profiled_traffic(args.workload, n, nq, k, chosen)
+
     line = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
@@ -302,7 +438,8 @@ def main():
                    "batch_latency_ms": round(wall / args.steps * 1e3, 4),
                    "sweep_ef_recall_qps": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_search", "kernel_ms": round(kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),

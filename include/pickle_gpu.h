@@ -145,8 +145,10 @@ int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const in
                         int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
                         int32_t *d_out_count, pk_query_counters *d_counters);
 
-/* Device failure flags of the last search (0 = none; 1 = a scratch capacity
- * was exceeded, 2 = a query had k or ef out of range).  Waits for the stream. */
+/* Device failure flags of the last search, or'ed over its queries (0 = none;
+ * 1 = a visited table filled up, 4 = a candidate heap filled up -- both
+ * scratch capacities; 2 = a query had k or ef out of range).  Waits for the
+ * stream. */
 int pk_device_error(pk_index *h, int *flags);
 
 /* Search launch shape: LDS candidate-heap entries, log2 of the slots of each
@@ -202,6 +204,12 @@ int pk_device_alloc(void **d_ptr, size_t bytes);
 int pk_device_free(void *d_ptr);
 int pk_copy_to_device(void *d_dst, const void *h_src, size_t bytes);
 int pk_copy_to_host(void *h_dst, const void *d_src, size_t bytes);
+
+/* Device-to-device copy on the handle's stream by a plain streaming kernel
+ * (k_copy16) that reads and writes exactly `bytes`; 16-byte aligned
+ * pointers and size.  No reference counterpart: it is the known-size kernel
+ * against which hardware byte counters are calibrated. */
+int pk_device_copy(pk_index *h, void *d_dst, const void *d_src, size_t bytes);
 
 /* HIP-event timing of the work enqueued between begin and end on the
  * handle's stream; pk_timer_end waits and returns milliseconds. */

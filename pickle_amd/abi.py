@@ -29,7 +29,7 @@ EXPORTS = [
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
-    "pk_index_insert",
+    "pk_index_insert", "pk_device_copy",
 ]
 BUILD_EXACT_ROWS = 1
 BRUTE_K = 32
@@ -84,6 +84,7 @@ def lib():
             "pk_sample_levels": (C.c_int, [i64, C.c_int, C.c_uint64, vp]),
             "pk_index_init_build": (C.c_int, [vp, vp]),
             "pk_index_insert": (C.c_int, [vp, vp, i64, C.c_int, C.c_int]),
+            "pk_device_copy": (C.c_int, [vp, vp, vp, sz]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -308,6 +309,9 @@ class Index:
         f = C.c_int()
         check(lib().pk_device_error(self.h, C.byref(f)))
         return f.value
+
+    def device_copy(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
+        check(lib().pk_device_copy(self.h, dst.ptr, src.ptr, nbytes))
 
     def timer_begin(self) -> None:
         check(lib().pk_timer_begin(self.h))

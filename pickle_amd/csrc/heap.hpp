@@ -48,10 +48,16 @@ struct WaveHeap {
     __device__ static __forceinline__ bool below(float x, float y) { return IS_MAX ? (x < y) : (x > y); }
 
     // per-lane access (any index); device-memory entries written by lane 0
-    // are read from L2, not from a possibly stale L1 line
+    // are read from L2, not from a possibly stale L1 line.  The device-memory
+    // branch waits for its own load before the paths join: otherwise the
+    // compiler places a vector-memory wait at the join, which the LDS path
+    // then also executes -- and that wait drains every older outstanding
+    // store of the wave (the visited-set log), on every heap access.
     __device__ __forceinline__ uint64_t raw_get(int i) const {
         if (!SPILL || i < lcap) return lds[i];
-        return __hip_atomic_load(&glob[i - lcap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t v = __hip_atomic_load(&glob[i - lcap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        return v;
     }
     __device__ __forceinline__ void raw_put(int i, uint64_t e) {
         if (!SPILL || i < lcap)

@@ -606,6 +606,33 @@ int pk_copy_to_host(void* d, const void* s, size_t bytes) {
     return PK_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// plain streaming copy of 16-byte units (the calibration kernel for HBM
+// byte counters: it reads and writes exactly `units` * 16 bytes)
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                 int64_t units) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < units; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int pk_device_copy(pk_index* h, void* d_dst, const void* d_src, size_t bytes) {
+    if (!h || !d_dst || !d_src) return fail(PK_ERR_ARG, "null argument");
+    if (bytes % 16 || (uintptr_t)d_dst % 16 || (uintptr_t)d_src % 16)
+        return fail(PK_ERR_ARG, "pk_device_copy needs 16-byte aligned pointers and sizes");
+    if (bytes == 0) return PK_OK;
+    int64_t units = (int64_t)(bytes / 16);
+    int grid = (int)std::min<int64_t>((units + 255) / 256, (int64_t)h->num_cus * 8);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, h->stream, static_cast<const uint4*>(d_src),
+                       static_cast<uint4*>(d_dst), units);
+    HIPCHK(hipGetLastError());
+    return PK_OK;
+}
+
 int pk_timer_begin(pk_index* h) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
     HIPCHK(hipEventRecord(h->ev0, h->stream));

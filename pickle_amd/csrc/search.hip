@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
         int k = uni(A.k ? A.k[q] : A.k_all);
         int ef = uni(A.ef ? A.ef[q] : A.ef_all);
         int fail = 0;
-        if (k < 1 || k > A.k_stride || ef < 1 || ef + 1 > A.wcap) fail = 2;
+        if (k < 1 || k > A.k_stride || ef < 1 || ef + 1 > A.wcap) fail = kFailArgs;
 
         int cur = 0;  // m_ent_id of the top layer (include/hnsw_layer.hpp:70, 186-188)
 

@@ -120,6 +120,12 @@ __device__ float one_distance(const DF& df, const LayerDev& L, int vstride, int 
     return unif(cd[0]);
 }
 
+// failure flags (pk_device_error): the visited table (1) or the candidate
+// heap (4) ran out of room; a query's k or ef was out of range (2)
+constexpr int kFailVisitedFull = 1;
+constexpr int kFailArgs = 2;
+constexpr int kFailHeapFull = 4;
+
 struct WorkCount {
     int hops = 0, neighbors = 0, dists = 0;
 };
@@ -195,7 +201,7 @@ __device__ void beam_level(const DF& df, const LayerDev& L, int vstride, int ent
                     const uint64_t e = pack_entry(dv, readlane(id, ln));
                     W.push_par(e, lane, leader);
                     if (C.full()) {
-                        fail = 1;
+                        fail = kFailHeapFull;
                         break;
                     }
                     C.push_par(e, lane, leader);

@@ -196,6 +196,38 @@ int pk_index_init_build(pk_index *h, const uint8_t *levels);
  * insert of all ids.  flags: PK_BUILD_EXACT_ROWS or 0.  Synchronous. */
 int pk_index_insert(pk_index *h, const int32_t *ext_ids, int64_t n, int max_batch, int flags);
 
+/* ---- precision form: low-precision traversal copy + exact FP32 re-rank ----
+ * No counterpart in the reference (its search reads one copy of the data).
+ * An index of dtype PK_FLOAT16 or PK_INT8 holds the traversal copy; these
+ * calls attach the FP32 rows and search with FP32 queries.  The search runs
+ * the ordinary batched AnnSearch (include/hnsw.hpp:509-541) on the copy with
+ * k = ef, then re-ranks those ef candidates by FP32 distances computed in the
+ * reference's own order (include/distance/avx512.hpp), ascending, ties by
+ * traversal rank; returned distances are the reference's FP32 values. */
+
+/* Per-dimension int8 quantizer of an int8 traversal copy:
+ * q[d] = clamp(rint((x[d] - offset[d]) * (1 / scale[d])), -127, 127). */
+int pk_index_set_quantizer(pk_index *h, const float *offset, const float *scale);
+
+/* FP32 rows [first, first + n) by external id (host, n x dim); the traversal
+ * copy of the same rows is converted on the device and becomes the index's
+ * base rows (replacing pk_index_set_base for this index). */
+int pk_index_set_full_rows(pk_index *h, const float *host_rows, int64_t first, int64_t n);
+
+/* Makes dst search the graph of src (same capacity, dim and M): layers are
+ * copied on the device and dst's layer vectors gathered from dst's rows.
+ * The copy is search-only (no insertion, no save). */
+int pk_index_copy_graph(pk_index *dst, pk_index *src);
+
+/* Precision-form search with FP32 queries (device buffers, nq x dim);
+ * ef >= k, one ef for all queries.  Outputs as pk_search_device. */
+int pk_search_rerank_device(pk_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
+                            int32_t k_stride, int32_t *d_out_ids, float *d_out_dists, int32_t *d_out_count,
+                            pk_query_counters *d_counters);
+/* The same with host buffers (copies in, runs, copies out). */
+int pk_search_rerank(pk_index *h, const float *queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,
+                     int32_t *out_ids, float *out_dists, int32_t *out_count, pk_query_counters *counters);
+
 /* Waits for all work enqueued on the handle's stream. */
 int pk_sync(pk_index *h);
 

@@ -29,7 +29,8 @@ EXPORTS = [
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
-    "pk_index_insert", "pk_device_copy",
+    "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
+    "pk_index_copy_graph", "pk_search_rerank_device", "pk_search_rerank",
 ]
 BUILD_EXACT_ROWS = 1
 BRUTE_K = 32
@@ -85,6 +86,11 @@ def lib():
             "pk_index_init_build": (C.c_int, [vp, vp]),
             "pk_index_insert": (C.c_int, [vp, vp, i64, C.c_int, C.c_int]),
             "pk_device_copy": (C.c_int, [vp, vp, vp, sz]),
+            "pk_index_set_quantizer": (C.c_int, [vp, vp, vp]),
+            "pk_index_set_full_rows": (C.c_int, [vp, vp, i64, i64]),
+            "pk_index_copy_graph": (C.c_int, [vp, vp]),
+            "pk_search_rerank_device": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_rerank": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -309,6 +315,39 @@ class Index:
         f = C.c_int()
         check(lib().pk_device_error(self.h, C.byref(f)))
         return f.value
+
+    # ---- precision form ----
+    def set_quantizer(self, offset: np.ndarray, scale: np.ndarray) -> None:
+        off = np.ascontiguousarray(offset, np.float32)
+        sc = np.ascontiguousarray(scale, np.float32)
+        check(lib().pk_index_set_quantizer(self.h, _p(off), _p(sc)))
+
+    def set_full_rows(self, rows: np.ndarray, first: int = 0) -> None:
+        rows = np.ascontiguousarray(rows, np.float32)
+        check(lib().pk_index_set_full_rows(self.h, _p(rows), first, rows.shape[0]))
+
+    def copy_graph(self, src: "Index") -> None:
+        check(lib().pk_index_copy_graph(self.h, src.h))
+
+    def search_rerank(self, queries: np.ndarray, k: int, ef: int, k_stride: int | None = None,
+                      counters: bool = False):
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        ks = k_stride or k
+        ids = np.empty((nq, ks), np.int32)
+        ds = np.empty((nq, ks), np.float32)
+        cnt = np.empty(nq, np.int32)
+        ctr = np.zeros(nq, COUNTER_DTYPE) if counters else None
+        check(lib().pk_search_rerank(self.h, _p(q), nq, k, ef, ks, _p(ids), _p(ds), _p(cnt),
+                                     _p(ctr) if counters else None))
+        return (ids, ds, cnt, ctr) if counters else (ids, ds, cnt)
+
+    def search_rerank_device(self, d_queries: "DeviceBuffer", nq: int, k: int, ef: int, k_stride: int,
+                             d_ids: "DeviceBuffer", d_dists: "DeviceBuffer", d_count: "DeviceBuffer" = None,
+                             d_counters: "DeviceBuffer" = None) -> None:
+        check(lib().pk_search_rerank_device(self.h, d_queries.ptr, nq, k, ef, k_stride, d_ids.ptr, d_dists.ptr,
+                                            d_count.ptr if d_count else None,
+                                            d_counters.ptr if d_counters else None))
 
     def device_copy(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
         check(lib().pk_device_copy(self.h, dst.ptr, src.ptr, nbytes))

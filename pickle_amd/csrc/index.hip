@@ -262,6 +262,10 @@ int pk_index_destroy(pk_index* h) {
     dfree(h->qcounter);
     dfree(h->err);
     for (auto& b : h->io) dfree(b.p);
+    for (auto& b : h->rr) dfree(b.p);
+    dfree(h->full);
+    dfree(h->q_offset);
+    dfree(h->q_inv_scale);
     (void)hipEventDestroy(h->ev0);
     (void)hipEventDestroy(h->ev1);
     (void)hipStreamDestroy(h->stream);
@@ -286,6 +290,25 @@ int pk_index_set_base(pk_index* h, const void* rows, int64_t first, int64_t n) {
     h->base_rows = std::max(h->base_rows, first + n);
     return PK_OK;
 }
+
+}  // extern "C"
+
+namespace pk {
+// the layer's own vector copy: base row ext[i] where has[i], zeros elsewhere
+int gather_layer_vectors(pk_index* h, int level) {
+    Layer& L = h->layers[level];
+    if (L.n > 0) {
+        int rows_per_block = 16;
+        int grid = (int)((L.n + rows_per_block - 1) / rows_per_block);
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, h->stream, L.vec, h->base, L.ext, L.has, L.n,
+                           h->vstride, h->capacity);
+        HIPCHK(hipGetLastError());
+    }
+    return PK_OK;
+}
+}  // namespace pk
+
+extern "C" {
 
 int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_len, const int32_t* ext,
                         const int32_t* degree, const int16_t* heur, const int32_t* adj, const float* dist,
@@ -320,13 +343,7 @@ int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_le
         HIPCHK(hipMemcpyAsync(L.dist, dist, (size_t)n * D * sizeof(float), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(L.ext, ext, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(L.has, has_vector, n, hipMemcpyHostToDevice, h->stream));
-    if (n > 0) {
-        int rows_per_block = 16;
-        int grid = (int)((n + rows_per_block - 1) / rows_per_block);
-        hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, h->stream, L.vec, h->base, L.ext, L.has, n,
-                           h->vstride, h->capacity);
-        HIPCHK(hipGetLastError());
-    }
+    if ((rc = gather_layer_vectors(h, level))) return rc;
     L.h_ext.assign(ext, ext + n);
     HIPCHK(hipStreamSynchronize(h->stream));
     L.loaded = true;

@@ -72,6 +72,7 @@ int publish_layers(pk_index* h);
 int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits);
 int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
                         int maxl);
+int gather_layer_vectors(pk_index* h, int level);
 }  // namespace pk
 
 struct pk_index {
@@ -106,4 +107,13 @@ struct pk_index {
     pk::IoBuf io[8];
     int last_grid = 0;
     size_t last_lds = 0;
+    // precision form: full-precision rows by external id (FP32) and the
+    // per-dimension int8 quantizer of the traversal copy
+    uint8_t* full = nullptr;
+    int full_vstride = 0;
+    int64_t full_rows = 0;
+    float* q_offset = nullptr;
+    float* q_inv_scale = nullptr;
+    bool has_quantizer = false;
+    pk::IoBuf rr[5];  // re-rank scratch: converted queries, candidate ids / dists / counts, counter
 };

@@ -1,0 +1,119 @@
+"""Precision form on the GPU: FP16 / per-dimension int8 traversal copies with
+an exact FP32 re-rank (pk_search_rerank), checked against the oracle.
+
+Pinning: the oracle graph is built on the FP32 vectors; the same graph is
+stored again with the traversal copy's vectors (FP16 by round-to-nearest-even,
+or the int8 quantizer restated in numpy), and the oracle's AnnSearch on it
+with k = ef gives the candidates.  The expected answer is those candidates
+ranked by the oracle's FP32 distance (the reference's summation order),
+ties by candidate rank.  Ids and distances must match bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pickle_amd import abi, datasets, precision
+from tests.gpu_util import index_from_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def quantize_i8(x, offset, scale):
+    """numpy restatement of the device quantizer (FP32 arithmetic)."""
+    inv = (np.float32(1.0) / scale.astype(np.float32)).astype(np.float32)
+    u = (x.astype(np.float32) - offset.astype(np.float32)).astype(np.float32)
+    u = (u * inv).astype(np.float32)
+    return np.clip(np.rint(u), -127, 127).astype(np.int8)
+
+
+def relayer(g32, conv, base_conv, dim):
+    """The oracle graph g32 with its layer vectors replaced by conv(rows);
+    rows without a vector stay zero, as on the device."""
+    g = O.Graph(base_conv, g32.metric, g32.M, g32.efc, g32.levels)
+    for l in range(g32.nlayers):
+        L = dict(g32.layer(l))
+        cap = int(L["cap"])
+        rows32 = np.frombuffer(np.ascontiguousarray(L["data"]).tobytes(), np.float32).reshape(-1, dim)[:cap]
+        conv_rows = conv(rows32)
+        conv_rows[L["has_data"][:cap] == 0] = 0
+        L["data"] = np.ascontiguousarray(conv_rows).view(np.uint8).reshape(-1)
+        g.set_layer(l, L)
+    g.set_enter_level(g32.enter_level)
+    return g
+
+
+def expected(gconv, qconv, queries, base, metric, k, ef):
+    cand, _, cnt = gconv.search(qconv, ef, ef)
+    ids = np.full((len(queries), k), -1, np.int32)
+    ds = np.full((len(queries), k), np.inf, np.float32)
+    for i in range(len(queries)):
+        c = cand[i, : cnt[i]]
+        d = np.array([O.distance(queries[i], base[j], metric) for j in c], np.float32)
+        order = np.lexsort((np.arange(len(c)), d))[:k]
+        ids[i, : len(order)] = c[order]
+        ds[i, : len(order)] = d[order]
+    return ids, ds
+
+
+@pytest.mark.parametrize("kind,metric", [("f16", O.L2), ("i8", O.L2), ("f16", O.IP), ("i8", O.IP)])
+def test_rerank_matches_oracle(kind, metric):
+    n, nq, dim, M, efc, k, ef = 3000, 64, 64, 12, 60, 10, 40
+    if metric == O.L2:
+        base, queries = datasets.gaussian_mixture(n, nq, dim, 16, 31, kind="f32")
+    else:
+        base, queries = datasets.vmf_mixture(n, nq, dim, 16, 31)
+    say("precision", kind, "L2" if metric == O.L2 else "IP", n, dim)
+    levels = O.sample_levels(n, M, seed=31)
+    g32 = O.Graph(base, metric, M, efc, levels)
+    g32.build(np.arange(n, dtype=np.int32))
+    if kind == "f16":
+        conv = lambda x: x.astype(np.float16)
+        base_c, q_c = base.astype(np.float16), queries.astype(np.float16)
+    else:
+        off, sc = precision.int8_quantizer(base, metric)
+        conv = lambda x: quantize_i8(x, off, sc)
+        base_c, q_c = quantize_i8(base, off, sc), quantize_i8(queries, off, sc)
+    gconv = relayer(g32, conv, base_c, dim)
+    want_ids, want_d = expected(gconv, q_c, queries, base, metric, k, ef)
+
+    src = index_from_oracle(g32, slack=False)  # the re-layered oracle graphs hold no slack row
+    idx = precision.traversal_index(src, base, kind)
+    ids, ds, cnt = idx.search_rerank(queries, k, ef)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(ds, want_d)
+    assert (cnt == k).all()
+    # the traversal copy alone, through the ordinary search, equals the oracle's
+    t_ids, t_d, _ = idx.search(q_c, 10, ef)
+    o_ids, o_d, _ = gconv.search(q_c, 10, ef)
+    np.testing.assert_array_equal(t_ids, o_ids)
+    np.testing.assert_array_equal(t_d, o_d)
+    idx.close()
+    src.close()
+
+
+def test_rerank_recall_and_errors():
+    n, nq, dim, M, efc = 20000, 300, 96, 16, 100
+    base, queries = datasets.gaussian_mixture(n, nq, dim, 32, 33, kind="f32")
+    gt, _ = O.brute_force(base, queries, 10, O.L2)
+    src = abi.Index(np.float32, abi.L2, dim, M, efc, n)
+    src.build(base, seed=33)
+    rec = lambda ids: np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, gt)])
+    r32 = rec(src.search(queries, 10, 64)[0])
+    for kind in ("f16", "i8"):
+        idx = precision.traversal_index(src, base, kind)
+        r = rec(idx.search_rerank(queries, 10, 64)[0])
+        say(f"  recall@10 ef=64: fp32 traversal {r32:.4f}, {kind} traversal + fp32 re-rank {r:.4f}")
+        assert r >= r32 - 0.01
+        with pytest.raises(abi.PickleError):
+            idx.search_rerank(queries, 10, 5)  # ef < k
+        idx.close()
+    with pytest.raises(abi.PickleError):
+        src.set_full_rows(base)  # an FP32 index has no separate traversal copy
+    src.close()

@@ -45,6 +45,9 @@ WORKLOADS = {
                          "M=32, efC=256, 10k queries, k=10, CPU-built graph", n=1_000_000, nq=10_000),
     "gm100k_f32": dict(desc="100k x 128 float32 (rounded), 64-cluster gaussian mixture (seed 0), L2, M=16, "
                             "efC=200, 1k queries, k=10", n=100_000, nq=1_000),
+    "gm10m_f32": dict(desc="10M x 128 float32, 64-cluster gaussian mixture (seed 3), L2, M=32, efC=200, 100k "
+                           "queries, k=10, GPU-built graph, fp16 traversal copy + FP32 re-rank",
+                      n=10_000_000, nq=100_000, build="gpu", precision="f16"),
 }
 
 
@@ -255,6 +258,14 @@ def main():
     ap.add_argument("--graph-cache", default=None, help="directory to reuse / keep the CPU-built graph")
     ap.add_argument("--no-cpu-search", action="store_true", help="skip the CPU search baseline")
     ap.add_argument("--cpu-repeat", type=int, default=3)
+    ap.add_argument("--cpu-queries", type=int, default=0, help="bound the CPU search sample (0 = all queries)")
+    ap.add_argument("--gt-queries", type=int, default=0, help="queries with ground truth (0 = all)")
+    ap.add_argument("--build", choices=["cpu", "gpu"], default=None,
+                    help="graph from the reference's CPU build or the GPU batched build (default per workload)")
+    ap.add_argument("--insert-batch", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precision", choices=["none", "f16", "i8"], default=None,
+                    help="search a low-precision traversal copy + FP32 re-rank (float workloads)")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",
                     help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")
     ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -294,41 +305,57 @@ def main():
     if binary is None:
         sys.exit("the reference build (oracle/_ref) is missing: run __graft_entry__.build() where the "
                  "reference sources are")
+    build = args.build or wl.get("build", "cpu")
+    prec = args.precision or wl.get("precision", "none")
+    if prec != "none" and base.dtype != np.float32:
+        sys.exit("--precision needs a float32 workload")
 
     work = args.graph_cache or tempfile.mkdtemp(prefix="pk_bench_")
     os.makedirs(work, exist_ok=True)
-    tag = f"{args.workload}_n{n}"
+    tag = f"{args.workload}_n{n}_{build}"
     base_path = os.path.join(work, f"{tag}.base")
     q_path = os.path.join(work, f"{tag}_q{nq}.queries")
     graph_path = os.path.join(work, f"{tag}.graph")
     meta_path = graph_path + ".build.json"
     base.tofile(base_path)
     queries.tofile(q_path)
+    mcode = abi.L2 if metric == "l2" else abi.IP
+    idx = abi.Index(base.dtype, mcode, dim, M, efc, n)
+    idx.set_base(base)
     if os.path.exists(graph_path) and os.path.exists(meta_path):
         build_info = json.load(open(meta_path))
         log(f"reusing graph {graph_path} (built in {build_info['build_seconds']:.1f}s)")
-    else:
+        idx.load_file(graph_path)
+    elif build == "cpu":
         log(f"reference CPU build ({isa}, {threads} threads) ...")
         build_info = cpu_build(binary, base_path, n, dim, dtype_name, metric, M, efc, threads, graph_path)
         json.dump(build_info, open(meta_path, "w"))
         log(f"built in {build_info['build_seconds']:.1f}s, {build_info['layers']} layers")
-
-    mcode = abi.L2 if metric == "l2" else abi.IP
-    idx = abi.Index(base.dtype, mcode, dim, M, efc, n)
-    idx.set_base(base)
-    idx.load_file(graph_path)
+        idx.load_file(graph_path)
+    else:
+        log(f"GPU batched build (insert batches up to {args.insert_batch}) ...")
+        tb = time.perf_counter()
+        idx.build(base, max_batch=args.insert_batch, seed=args.seed)
+        idx.sync()
+        build_info = {"build_seconds": time.perf_counter() - tb, "threads": 0, "where": "gpu",
+                      "insert_batch": args.insert_batch}
+        log(f"GPU build in {build_info['build_seconds']:.1f}s")
+        idx.save_file(graph_path)  # for the reference's search on the same graph
+        json.dump(build_info, open(meta_path, "w"))
     info = idx.info()
     log(f"graph in HBM: layers {info['sizes']}")
 
+    gt_nq = min(nq, args.gt_queries) if args.gt_queries else nq
     t1 = time.time()
-    gt_ids, gt_d = idx.brute_force(queries)
+    gt_ids, gt_d = idx.brute_force(queries[:gt_nq])
     if base.dtype.kind == "f":  # FP64 re-rank of the FP32 candidates
-        A, Q = base.astype(np.float64), queries.astype(np.float64)
-        for i in range(nq):
+        A, Q = base.astype(np.float64), queries[:gt_nq].astype(np.float64)
+        for i in range(gt_nq):
             c = gt_ids[i]
             d64 = ((A[c] - Q[i]) ** 2).sum(1) if metric == "l2" else -(A[c] @ Q[i])
             gt_ids[i] = c[np.lexsort((c, d64))]
-    log(f"ground truth (GPU brute force) in {time.time() - t1:.1f}s")
+        del A
+    log(f"ground truth (GPU brute force, {gt_nq} queries) in {time.time() - t1:.1f}s")
 
     # HBM-resident buffers for the timed runs
     d_q = abi.DeviceBuffer.from_array(queries)
@@ -336,65 +363,104 @@ def main():
     d_dist = abi.DeviceBuffer(nq * k * 4)
     d_cnt = abi.DeviceBuffer(nq * 4)
     d_ctr = abi.DeviceBuffer(nq * abi.COUNTER_DTYPE.itemsize)
+    tidx = None
+    if prec != "none":
+        from pickle_amd import precision
 
-    def run_once(ef, counters=False):
-        idx.search_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt, d_ctr if counters else None)
+        tp = time.perf_counter()
+        tidx = precision.traversal_index(idx, base, prec)
+        log(f"{prec} traversal copy + FP32 rows in HBM ({time.perf_counter() - tp:.1f}s)")
 
-    sweep = []
+    def backend(which):
+        if which == "fp32":
+            return idx, (lambda ef, c=False: idx.search_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt,
+                                                               d_ctr if c else None))
+        return tidx, (lambda ef, c=False: tidx.search_rerank_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt,
+                                                                    d_ctr if c else None))
+
+    def sweep(which, target, efs):
+        h, run = backend(which)
+        out, chosen = [], None
+        for ef in efs:
+            if ef < k:
+                continue
+            run(ef)
+            h.sync()
+            h.timer_begin()
+            run(ef)
+            ms = h.timer_end()
+            ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
+            r = recall_at_k(ids, gt_ids, k)
+            out.append([ef, round(r, 4), round(nq / (ms / 1e3), 1)])
+            log(f"[{which}] ef={ef:4d} recall@{k}={r:.4f} qps={nq / (ms / 1e3):,.0f}")
+            if r >= target and chosen is None:
+                chosen = ef
+                if not args.ef:
+                    break
+        if chosen is None:
+            chosen = out[-1][0]
+            log(f"[{which}] no ef reached recall {target}; using ef={chosen}")
+        return chosen, [x[1] for x in out if x[0] == chosen][0], out
+
+    def timed(which, ef):
+        h, run = backend(which)
+        for _ in range(args.warmup):
+            run(ef)
+        h.sync()
+        tw0 = time.perf_counter()
+        h.timer_begin()
+        for _ in range(args.steps):
+            run(ef)
+        ev_ms = h.timer_end()
+        h.sync()
+        return time.perf_counter() - tw0, ev_ms / 1e3 / args.steps
+
     efs = [args.ef] if args.ef else [int(e) for e in args.efs.split(",")]
-    chosen = None
-    for ef in efs:
-        run_once(ef)
-        idx.sync()
-        idx.timer_begin()
-        run_once(ef)
-        ms = idx.timer_end()
-        ids = d_ids.to_array(np.int32, (nq, k))
-        r = recall_at_k(ids, gt_ids, k)
-        sweep.append([ef, round(r, 4), round(nq / (ms / 1e3), 1)])
-        log(f"ef={ef:4d} recall@{k}={r:.4f} qps={nq / (ms / 1e3):,.0f}")
-        if r >= args.recall_target and chosen is None:
-            chosen = ef
-            if not args.ef:
-                break
-    if chosen is None:
-        chosen = efs[-1]
-        log(f"no ef reached recall {args.recall_target}; using ef={chosen}")
-    recall = [s[1] for s in sweep if s[0] == chosen][0]
-    if idx.device_error():
+    comparison = None
+    if prec == "none":
+        main_be = "fp32"
+        chosen, recall, sweep_out = sweep("fp32", args.recall_target, efs)
+    else:
+        ef32, r32, sw32 = sweep("fp32", args.recall_target, efs)
+        wall32, ks32 = timed("fp32", ef32)
+        qps32 = nq * args.steps / wall32
+        # the precision form at equal recall: the first ef within 0.002 of the FP32 recall
+        main_be = "rerank"
+        chosen, recall, sweep_out = sweep("rerank", r32 - 0.002, efs)
+        comparison = {"fp32_traversal": {"ef": ef32, "recall@10": r32, "qps": round(qps32, 1),
+                                         "kernel_ms": round(ks32 * 1e3, 4), "sweep_ef_recall_qps": sw32}}
+    h_main, run_main = backend(main_be)
+    if h_main.device_error():
         sys.exit("device reported a search failure")
 
     # work counters (untimed pass, as the reference's second profiler pass)
-    run_once(chosen, counters=True)
-    idx.sync()
+    run_main(chosen, True)
+    h_main.sync()
     ctr = d_ctr.to_array(abi.COUNTER_DTYPE, (nq,))
-    elem = base.dtype.itemsize
+    elem = {"none": base.dtype.itemsize, "f16": 2, "i8": 1}[prec]
     hops = ctr["hops"].sum(axis=1).astype(np.int64)
     nbr = ctr["neighbors"].sum(axis=1).astype(np.int64)
     dists = ctr["dists"].sum(axis=1).astype(np.int64)
     # algorithmic bytes (reference Profiler units): 4 B per adjacency entry
-    # scanned, 8 B of node metadata per hop, one row per distance, the query
+    # scanned, 8 B of node metadata per hop, one row per distance, the query;
+    # the precision form adds the FP32 rows of the ef re-ranked candidates
     alg_bytes = int((4 * nbr + 8 * hops + elem * dim * dists).sum() + nq * elem * dim)
+    if prec != "none":
+        alg_bytes += nq * (chosen * (4 * dim + 4) + 4 * dim)
     khz = abi.wall_clock_khz()
     lat_ms = ctr["latency_ticks"].astype(np.float64) / khz
 
-    # timed region
-    for _ in range(args.warmup):
-        run_once(chosen)
-    idx.sync()
-    tw0 = time.perf_counter()
-    idx.timer_begin()
-    for _ in range(args.steps):
-        run_once(chosen)
-    ev_ms = idx.timer_end()
-    idx.sync()
-    wall = time.perf_counter() - tw0
+    wall, kernel_s = timed(main_be, chosen)
     qps = nq * args.steps / wall
-    kernel_s = ev_ms / 1e3 / args.steps
     achieved = alg_bytes / kernel_s / 1e9
-    grid, lds = idx.last_launch()
+    grid, lds = h_main.last_launch()
     log(f"timed: {args.steps} x {nq} queries at ef={chosen}: {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
+    if comparison:
+        comparison["speedup_vs_fp32_traversal"] = round(qps / comparison["fp32_traversal"]["qps"], 3)
+        log(f"{prec} traversal + FP32 re-rank: {qps:,.0f} QPS at recall {recall:.4f} vs FP32 traversal "
+            f"{comparison['fp32_traversal']['qps']:,.0f} QPS at {comparison['fp32_traversal']['recall@10']:.4f}"
+            f" -> {comparison['speedup_vs_fp32_traversal']:.3f}x")
 
     # known-size copy (k_copy16, 256 MiB read + 256 MiB written, twice):
     # the kernel against which profiled HBM byte counters are calibrated
@@ -406,42 +472,61 @@ def main():
     calib.free()
     calib2.free()
 
-    # cpu_baseline leg: the reference's AnnSearch on the host cores
+    # cpu_baseline leg: the reference's AnnSearch on the host cores, same
+    # graph, FP32 data, the FP32 operating point
     cpu = None
     if not args.no_cpu_search:
+        cpu_ef = comparison["fp32_traversal"]["ef"] if comparison else chosen
+        cnq = min(nq, args.cpu_queries) if args.cpu_queries else nq
+        if cnq < nq:
+            queries[:cnq].tofile(q_path + ".sample")
+            cq_path = q_path + ".sample"
+        else:
+            cq_path = q_path
         out = os.path.join(work, "cpu_res")
-        st = cpu_search(binary, graph_path, base_path, q_path, n, nq, dim, dtype_name, k, chosen, threads,
+        st = cpu_search(binary, graph_path, base_path, cq_path, n, cnq, dim, dtype_name, k, cpu_ef, threads,
                         args.cpu_repeat, out)
-        cpu_ids = read_ref_results(f"{out}.ef{chosen}", nq, k)
-        cpu_recall = recall_at_k(cpu_ids, gt_ids, k)
-        same = float(np.mean(np.all(cpu_ids == d_ids.to_array(np.int32, (nq, k)), axis=1)))
-        cpu = {"value": round(st["qps"], 1), "unit": "queries/s", "cores": threads, "kind": "reference",
-               "sample": f"all {nq} queries, ef={chosen}, k={k}, best of {args.cpu_repeat} runs, reference "
-                         f"AnnSearch compiled from source ({isa} distance path) on {cpu_model()}",
-               "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
-               "ids_identical_to_gpu": round(same, 4), "build_seconds": round(build_info["build_seconds"], 2),
-               "build_threads": build_info["threads"]}
-        log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads, recall {cpu_recall:.4f}, "
-            f"GPU ids identical on {same * 100:.2f}% of queries")
+        cpu_ids = read_ref_results(f"{out}.ef{cpu_ef}", cnq, k)
+        r_n = min(cnq, gt_nq)
+        cpu_recall = recall_at_k(cpu_ids[:r_n], gt_ids[:r_n], k)
+        entry = {"value": round(st["qps"], 1), "unit": "queries/s", "cores": threads, "kind": "reference",
+                 "sample": f"{cnq} queries, ef={cpu_ef}, k={k}, best of {args.cpu_repeat} runs, reference "
+                           f"AnnSearch compiled from source ({isa} distance path) on {cpu_model()}, "
+                           f"{'reference-built' if build == 'cpu' else 'GPU-built'} graph",
+                 "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
+                 "build_seconds": round(build_info["build_seconds"], 2),
+                 "build_threads": build_info["threads"], "build_where": build_info.get("where", "cpu")}
+        if not comparison:
+            same = float(np.mean(np.all(cpu_ids == d_ids.to_array(np.int32, (nq, k))[:cnq], axis=1)))
+            entry["ids_identical_to_gpu"] = round(same, 4)
+            log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads, recall {cpu_recall:.4f}, "
+                f"GPU ids identical on {same * 100:.2f}% of queries")
+        else:
+            log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads at ef={cpu_ef}, recall {cpu_recall:.4f}")
+        cpu = entry
 
-    traffic, traffic_src = profiled_traffic(args.workload, n, nq, k, chosen)
-
+    traffic, traffic_src = profiled_traffic(args.workload, n, nq, k, chosen) if prec == "none" else (None, None)
+    search_desc = "exact (reference heap order)" if prec == "none" else \
+        f"{prec} traversal copy + exact FP32 re-rank of the ef candidates"
+    dtype_out = dtype_name if prec == "none" else f"{prec} traversal, f32 re-rank"
+    cfg = {"workload": wl["desc"] if (n, nq) == (wl["n"], wl["nq"]) else f"{wl['desc']} (n={n}, nq={nq})",
+           "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc, "k": k, "ef": chosen, "recall@10": recall,
+           "recall_queries": gt_nq, "graph": "reference CPU build" if build == "cpu" else
+           f"GPU batched build ({build_info['build_seconds']:.1f}s)", "search": search_desc,
+           "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
+           "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
+           "batch_latency_ms": round(wall / args.steps * 1e3, 4), "sweep_ef_recall_qps": sweep_out}
+    if comparison:
+        cfg["precision_comparison"] = comparison
     line = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": dtype_name, "data": "synthetic",
-        "config": {"workload": wl["desc"] if (n, nq) == (wl["n"], wl["nq"]) else f"{wl['desc']} (n={n}, nq={nq})",
-                   "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc, "k": k, "ef": chosen,
-                   "recall@10": recall, "search": "exact (reference heap order)",
-                   "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
-                   "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
-                   "batch_latency_ms": round(wall / args.steps * 1e3, 4),
-                   "sweep_ef_recall_qps": sweep},
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype_out, "data": "synthetic", "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "k_search", "kernel_ms": round(kernel_s * 1e3, 4),
-                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel": "k_search" if prec == "none" else "k_quantize + k_search + k_rerank",
+                     "kernel_ms": round(kernel_s * 1e3, 4), "algorithmic_bytes_per_launch": alg_bytes,
                      "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),
                                    "dists": round(float(dists.mean()), 2)}},
         "cpu_baseline": cpu,

@@ -56,11 +56,14 @@ struct IntDist {
         qq = sum8((int)part);
     }
 
-    __device__ void load_row(Row& r, const uint8_t* row, int g, bool active) const {
+    // Loads are issued unconditionally (chunks past the row are read from
+    // its first chunk and ignored in finish): no branch separates them, so
+    // all of a batch's gathers are in flight together.
+    __device__ void load_row(Row& r, const uint8_t* row, int g, bool /*active*/) const {
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
-            int c = g + 8 * i;
-            r.c[i] = (active && c < nchunks) ? ld16_global(row + 16 * c) : make_uint4(0, 0, 0, 0);
+            const int c = g + 8 * i;
+            r.c[i] = ld16_global(row + 16 * (c < nchunks ? c : 0));
         }
     }
 
@@ -69,6 +72,7 @@ struct IntDist {
         uint32_t ab = 0, bb = 0;
 #pragma unroll
         for (int i = 0; i < NCH; i++) {
+            if (g + 8 * i >= nchunks) continue;
             ab = dot4(SIGNED, q[i].x, r.c[i].x, ab);
             ab = dot4(SIGNED, q[i].y, r.c[i].y, ab);
             ab = dot4(SIGNED, q[i].z, r.c[i].z, ab);
@@ -116,7 +120,7 @@ struct FloatDist {
     static constexpr int LPC = 16;
     static constexpr bool HALF = DT == DT_F16;
     struct Row {
-        float e[NEL];
+        uint32_t raw[NEL];  // element bits (binary16 in the low half for fp16)
     };
     float q[NEL];
     int nmain;  // dim / 16
@@ -131,9 +135,21 @@ struct FloatDist {
         for (int i = 0; i < NEL; i++) q[i] = i < nmain ? qelem<HALF>(qst, 16 * i + g) : 0.0f;
     }
 
-    __device__ void load_row(Row& r, const uint8_t* row, int g, bool active) const {
+    // unconditional loads of element bits (elements past the row re-read
+    // element g); conversion waits until finish, so all loads are in flight
+    __device__ void load_row(Row& r, const uint8_t* row, int g, bool /*active*/) const {
 #pragma unroll
-        for (int i = 0; i < NEL; i++) r.e[i] = (active && i < nmain) ? felem<HALF>(row, 16 * i + g) : 0.0f;
+        for (int i = 0; i < NEL; i++) {
+            const int e = i < nmain ? 16 * i + g : g;
+            if (HALF)
+                r.raw[i] = gbl(reinterpret_cast<const uint16_t*>(row))[e];
+            else
+                r.raw[i] = gbl(reinterpret_cast<const uint32_t*>(row))[e];
+        }
+    }
+    __device__ static __forceinline__ float elem(uint32_t bits) {
+        if (HALF) return (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
+        return __uint_as_float(bits);
     }
 
     // sequential tail in the compiled reference's order (first lane only)
@@ -169,7 +185,7 @@ struct FloatDist {
         float acc = 0.0f;
 #pragma unroll
         for (int i = 0; i < NEL; i++)
-            if (i < nmain) acc = fterm<MET>(q[i], r.e[i], acc);
+            if (i < nmain) acc = fterm<MET>(q[i], elem(r.raw[i]), acc);
         float s = tree_sum16(acc);
         if (rem != 0 && g == 0 && active) s = s + tail(row);
         return MET == MET_IP ? -s : s;

@@ -169,14 +169,16 @@ int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int wa
         if ((rc = dalloc(&s.cglob, (size_t)waves * h->cg_cap)) || (rc = dalloc(&s.vtab, slots)) ||
             (rc = dalloc(&s.vlog, slots)))
             return rc;
-        HIPCHK(hipMemset(s.vtab, 0, slots * sizeof(uint32_t)));
+        HIPCHK(hipMemsetAsync(s.vtab, 0, slots * sizeof(uint32_t), h->stream));
         s.waves = waves;
         s.vbits = vbits;
     }
     if (!s.counter) {
         int rc;
-        if ((rc = dalloc(&s.counter, 1)) || (rc = dalloc(&s.err, 1))) return rc;
-        HIPCHK(hipMemset(s.err, 0, 4));
+        // err[0]: flags; err[1..4]: the first failing node (ext id, level,
+        // ids in its visited table, hops) for the error message
+        if ((rc = dalloc(&s.counter, 1)) || (rc = dalloc(&s.err, 8))) return rc;
+        HIPCHK(hipMemsetAsync(s.err, 0, 32, h->stream));
     }
     return PK_OK;
 }

@@ -141,6 +141,8 @@ __global__ __launch_bounds__(kInsBlock) void k_insert_search(InsertArgs A) {
         df.load_query(qst, g, qdim);
         int fail = 0;
         WorkCount wc;
+        vis.peak = 0;
+        vis.full_at = -1;
         int cur = 0;  // m_ent_id of the top layer
         for (int lev = A.enter_level; lev >= l + 1 && !fail; lev--) {
             if (!((A.nonempty >> lev) & 1)) continue;
@@ -212,7 +214,16 @@ __global__ __launch_bounds__(kInsBlock) void k_insert_search(InsertArgs A) {
             cur = uni(rid[0]);
             if (lev > 0) cur = L.down[cur];
         }
-        if (fail && leader) atomicOr(A.err, fail);
+        if (fail && leader) {
+            atomicOr(A.err, fail);
+            if (atomicCAS(&A.err[1], 0, 1) == 0) {
+                A.err[2] = ext;
+                A.err[3] = l;
+                A.err[4] = vis.peak;
+                A.err[5] = wc.hops;
+                A.err[6] = vis.full_at;
+            }
+        }
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -312,14 +323,17 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
                          uint64_t* h2, float* cd, uint8_t* qst, int& fail) {
     const bool leader = lane == 0;
     if (!lock_acquire(&B.lock[vid], lane)) {
-        fail = 1;
+        fail = kFailLockTimeout;
         return;
     }
     const size_t rowbase = (size_t)vid * B.D;
-    const int dg = uni(B.deg[vid]);
+    // a row that grew into the end of the layer's allocation holds what fit
+    // (the reference's flat array has no room there either)
+    int dg = uni(B.deg[vid]);
+    if ((size_t)dg > (size_t)B.adj_len - rowbase) dg = (int)(B.adj_len - rowbase);
     int h = uni((int)B.heur[vid]);
     if (h > dg) h = dg;
-    if (dg > kRowMax - 1 || rowbase + dg > (size_t)B.adj_len) {
+    if (dg > kRowMax - 1) {
         fail = 1;
         lock_release(&B.lock[vid], lane);
         return;
@@ -332,8 +346,11 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
         rb_d[j] = B.dist[rowbase + j];
     }
     __builtin_amdgcn_wave_barrier();
-    if (ballot(bad)) {  // a row holding an id outside the layer: leave it alone
-        fail = 1;
+    if (ballot(bad)) {
+        // a row that reads past its own slots into storage no node has
+        // written yet (ids outside the layer): the reference would compute
+        // distances to such ids; here the edge is dropped and the row left
+        // as it is
         lock_release(&B.lock[vid], lane);
         return;
     }
@@ -374,9 +391,10 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
                 B.deg[vid] = dg + (dg < B.D);
             }
         }
-    } else if (!A.exact_rows && offset >= B.D) {
+    } else if ((!A.exact_rows && offset >= B.D) || offset >= kRowMax - 1) {
         // a full row of closer heuristic neighbours: the reference writes
-        // slot D (hnsw_layer.hpp:420, 441-447); rows capped at D drop the edge
+        // slot D (hnsw_layer.hpp:420, 441-447); rows capped at D drop the
+        // edge, and so do reference rows at their limit of kRowMax - 1 ids
     } else {
         // re-run the heuristic on the entries from offset on (:409-463)
         WaveHeap<false, false> W{h1, h1, kRowMax, 0, 0};
@@ -433,8 +451,8 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
             B.dist[rowbase + j] = nb_d[j];
         }
         if (leader) {
-            B.heur[vid] = (int16_t)cur_h;
-            B.deg[vid] = cur_deg;
+            B.heur[vid] = (int16_t)(cur_h < wr ? cur_h : wr);
+            B.deg[vid] = wr;
         }
     }
     lock_release(&B.lock[vid], lane);
@@ -493,40 +511,45 @@ __global__ void k_fill_f32_ins(float* p, float v, int64_t n) {
 }
 
 template <class DF, bool INT>
-static hipError_t launch_ins(const InsertArgs& a, int grid, size_t lds_a, size_t lds_c, hipStream_t s) {
+// phase 0: construction searches (R / Pruned lists to scratch only, so the
+// phase can run again); phase 1: AddNode rows, then AddEdge
+static hipError_t launch_ins(const InsertArgs& a, int phase, int grid, size_t lds_a, size_t lds_c, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32_ins, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
-    hipLaunchKernelGGL((k_insert_search<DF, INT>), dim3(grid), dim3(kInsBlock), lds_a, s, a);
-    hipLaunchKernelGGL(k_fill_i32_ins, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
-    hipLaunchKernelGGL(k_insert_rows, dim3(grid), dim3(kInsBlock), 0, s, a);
-    hipLaunchKernelGGL(k_fill_i32_ins, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
-    hipLaunchKernelGGL((k_insert_edges<DF, INT>), dim3(grid), dim3(kInsBlock), lds_c, s, a);
+    if (phase == 0) {
+        hipLaunchKernelGGL((k_insert_search<DF, INT>), dim3(grid), dim3(kInsBlock), lds_a, s, a);
+    } else {
+        hipLaunchKernelGGL(k_insert_rows, dim3(grid), dim3(kInsBlock), 0, s, a);
+        hipLaunchKernelGGL(k_fill_i32_ins, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
+        hipLaunchKernelGGL((k_insert_edges<DF, INT>), dim3(grid), dim3(kInsBlock), lds_c, s, a);
+    }
     return hipGetLastError();
 }
 
 template <int DT, int MET>
-static hipError_t launch_ins_dt(const InsertArgs& a, int size_class, int grid, size_t la, size_t lc, hipStream_t s) {
+static hipError_t launch_ins_dt(const InsertArgs& a, int phase, int size_class, int grid, size_t la, size_t lc,
+                                hipStream_t s) {
     constexpr bool INT = DT == DT_U8 || DT == DT_I8;
     if constexpr (INT) {
-        if (size_class == 0) return launch_ins<IntDist<DT, MET, 1>, true>(a, grid, la, lc, s);
-        if (size_class == 1) return launch_ins<IntDist<DT, MET, 2>, true>(a, grid, la, lc, s);
-        return launch_ins<IntDist<DT, MET, 4>, true>(a, grid, la, lc, s);
+        if (size_class == 0) return launch_ins<IntDist<DT, MET, 1>, true>(a, phase, grid, la, lc, s);
+        if (size_class == 1) return launch_ins<IntDist<DT, MET, 2>, true>(a, phase, grid, la, lc, s);
+        return launch_ins<IntDist<DT, MET, 4>, true>(a, phase, grid, la, lc, s);
     } else {
-        if (size_class == 0) return launch_ins<FloatDist<DT, MET, 8>, false>(a, grid, la, lc, s);
-        return launch_ins<FloatDist<DT, MET, 16>, false>(a, grid, la, lc, s);
+        if (size_class == 0) return launch_ins<FloatDist<DT, MET, 8>, false>(a, phase, grid, la, lc, s);
+        return launch_ins<FloatDist<DT, MET, 16>, false>(a, phase, grid, la, lc, s);
     }
 }
 
-static hipError_t launch_insert(const InsertArgs& a, int dtype, int metric, int size_class, int grid, size_t la,
-                                size_t lc, hipStream_t s) {
+static hipError_t launch_insert(const InsertArgs& a, int phase, int dtype, int metric, int size_class, int grid,
+                                size_t la, size_t lc, hipStream_t s) {
     switch (dtype * 4 + metric) {
-        case DT_U8 * 4 + MET_L2: return launch_ins_dt<DT_U8, MET_L2>(a, size_class, grid, la, lc, s);
-        case DT_U8 * 4 + MET_IP: return launch_ins_dt<DT_U8, MET_IP>(a, size_class, grid, la, lc, s);
-        case DT_I8 * 4 + MET_L2: return launch_ins_dt<DT_I8, MET_L2>(a, size_class, grid, la, lc, s);
-        case DT_I8 * 4 + MET_IP: return launch_ins_dt<DT_I8, MET_IP>(a, size_class, grid, la, lc, s);
-        case DT_F32 * 4 + MET_L2: return launch_ins_dt<DT_F32, MET_L2>(a, size_class, grid, la, lc, s);
-        case DT_F32 * 4 + MET_IP: return launch_ins_dt<DT_F32, MET_IP>(a, size_class, grid, la, lc, s);
-        case DT_F16 * 4 + MET_L2: return launch_ins_dt<DT_F16, MET_L2>(a, size_class, grid, la, lc, s);
-        case DT_F16 * 4 + MET_IP: return launch_ins_dt<DT_F16, MET_IP>(a, size_class, grid, la, lc, s);
+        case DT_U8 * 4 + MET_L2: return launch_ins_dt<DT_U8, MET_L2>(a, phase, size_class, grid, la, lc, s);
+        case DT_U8 * 4 + MET_IP: return launch_ins_dt<DT_U8, MET_IP>(a, phase, size_class, grid, la, lc, s);
+        case DT_I8 * 4 + MET_L2: return launch_ins_dt<DT_I8, MET_L2>(a, phase, size_class, grid, la, lc, s);
+        case DT_I8 * 4 + MET_IP: return launch_ins_dt<DT_I8, MET_IP>(a, phase, size_class, grid, la, lc, s);
+        case DT_F32 * 4 + MET_L2: return launch_ins_dt<DT_F32, MET_L2>(a, phase, size_class, grid, la, lc, s);
+        case DT_F32 * 4 + MET_IP: return launch_ins_dt<DT_F32, MET_IP>(a, phase, size_class, grid, la, lc, s);
+        case DT_F16 * 4 + MET_L2: return launch_ins_dt<DT_F16, MET_L2>(a, phase, size_class, grid, la, lc, s);
+        case DT_F16 * 4 + MET_IP: return launch_ins_dt<DT_F16, MET_IP>(a, phase, size_class, grid, la, lc, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -655,7 +678,10 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
     for (int64_t t = 0; t < n; t++)
         if (ext_ids[t] < 0 || ext_ids[t] >= h->capacity || ext_ids[t] >= h->base_rows)
             return set_error(PK_ERR_ARG, "ext id " + std::to_string(ext_ids[t]) + " outside the base rows");
-    const int vbits = 15;
+    // visited table: room for the efc-bounded search's visits at a low load;
+    // grown (up to 2^20 slots per wave) when a search fills it, since rows
+    // that grew past D make some searches touch many more ids
+    int vbits = std::max(h->ins.vbits, h->efc <= 64 ? 14 : (h->efc <= 160 ? 15 : 16));
     const int wcap = (h->efc + 2 + 7) / 8 * 8;
     if (wcap > 2 * 64 * kPopSlots) return set_error(PK_ERR_ARG, "efc too large (max 630)");
     const int ccap = 1024;
@@ -674,6 +700,20 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
     const int topk = 2 * h->M;
     int rc = pk::ensure_insert_scratch(h, max_batch, maxl_all, topk, grid_cap, vbits);
     if (rc) return rc;
+    int32_t diag[8] = {0};
+    auto device_flags = [&](int32_t* out) -> int {
+        ICHK(hipStreamSynchronize(h->stream));
+        ICHK(hipMemcpy(diag, h->ins.err, sizeof(diag), hipMemcpyDeviceToHost));
+        *out = diag[0];
+        return PK_OK;
+    };
+    auto diag_text = [&]() {
+        return " (flags " + std::to_string(diag[0]) + "; first failing node: ext " + std::to_string(diag[2]) +
+               ", level " + std::to_string(diag[3]) + ", " + std::to_string(diag[4]) + " visited ids, " +
+               std::to_string(diag[5]) + " hops, table full after " + std::to_string(diag[6]) +
+               " inserts, visited table 2^" + std::to_string(vbits) + ", recorded " + std::to_string(diag[1]) + ")";
+    };
+    ICHK(hipMemsetAsync(h->ins.err, 0, 32, h->stream));
 
     std::vector<int32_t> b_ext, b_lvl, b_in;
     int64_t t = 0;
@@ -745,7 +785,28 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
         a.qcounter = h->ins.counter;
         a.err = h->ins.err;
         const int grid = (int)std::min<int64_t>(nb, grid_cap);
-        ICHK(launch_insert(a, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+        for (;;) {
+            ICHK(launch_insert(a, 0, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+            int32_t err = 0;
+            if ((rc = device_flags(&err))) return rc;
+            if (err == 0) break;
+            if (err != kFailVisitedFull || vbits >= 20)
+                return set_error(PK_ERR_DEVICE, "insertion search failed on the device" + diag_text());
+            vbits++;
+            if ((rc = pk::ensure_insert_scratch(h, max_batch, maxl_all, topk, grid_cap, vbits))) return rc;
+            a.vtab = h->ins.vtab;
+            a.vlog = h->ins.vlog;
+            a.cglob = h->ins.cglob;
+            a.vg_bits = vbits;
+            ICHK(hipMemsetAsync(h->ins.err, 0, 32, h->stream));
+        }
+        ICHK(launch_insert(a, 1, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+        {
+            int32_t err = 0;
+            if ((rc = device_flags(&err))) return rc;
+            if (err)
+                return set_error(PK_ERR_DEVICE, "AddEdge failed on the device (flags " + std::to_string(err) + ")");
+        }
         // SetEntLevel (hnsw.hpp:146-151) and the new sizes become visible
         for (int64_t k = 0; k < nb; k++) h->enter_level = std::max(h->enter_level, b_lvl[k]);
         if ((rc = pk::publish_layers(h))) return rc;

@@ -27,7 +27,7 @@
 namespace pk {
 
 template <class DF, bool IS_INT>
-__global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
+__global__ __launch_bounds__(kSearchBlock, 4) void k_search(SearchArgs A) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;

@@ -45,12 +45,19 @@ __device__ void batch_distances(const DF& df, const uint8_t* vec, int vstride, c
         typename DF::Row rows[PB];
         const uint8_t* rp[PB];
         bool act[PB];
+        int vid[PB];
+        // all candidate ids first (one LDS round trip), then every row
+        // gather without a branch between them; lanes past ncand re-read
+        // the first candidate's row and their distances are dropped
 #pragma unroll
         for (int p = 0; p < PB; p++) {
-            int ci = base + p * G + grp;
+            const int ci = base + p * G + grp;
             act[p] = ci < ncand;
-            int vid = act[p] ? cl[ci] : 0;
-            rp[p] = vec + (size_t)vid * (size_t)vstride;
+            vid[p] = cl[act[p] ? ci : 0];
+        }
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            rp[p] = vec + (size_t)vid[p] * (size_t)vstride;
             df.load_row(rows[p], rp[p], g, act[p]);
         }
 #pragma unroll
@@ -125,6 +132,7 @@ __device__ float one_distance(const DF& df, const LayerDev& L, int vstride, int 
 constexpr int kFailVisitedFull = 1;
 constexpr int kFailArgs = 2;
 constexpr int kFailHeapFull = 4;
+constexpr int kFailLockTimeout = 8;  // insertion: a node lock was held for seconds
 
 struct WorkCount {
     int hops = 0, neighbors = 0, dists = 0;

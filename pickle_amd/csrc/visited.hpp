@@ -26,6 +26,8 @@ struct WaveVisited {
     uint32_t mask;
     int bits;
     int count;      // slots filled by the current search (uniform)
+    int peak = 0;        // largest count seen at a clear (diagnostics)
+    int full_at = -1;    // count when a probe found the table full (diagnostics)
     uint32_t tag;   // chunk counter
 
     __device__ __forceinline__ uint32_t home(int id) const {
@@ -75,7 +77,10 @@ struct WaveVisited {
                 }
             }
         }
-        if (ballot(r == 3)) fail = 1;
+        if (ballot(r == 3)) {
+            fail = 1;
+            full_at = count;
+        }
         uint64_t newm = ballot(r == 0);
         uint64_t dupm = ballot(r == 2);
         if (newm) {
@@ -109,6 +114,7 @@ struct WaveVisited {
     __device__ void clear(int lane) {
         // log entries were written by other lanes: read them from L2
         for (int i = lane; i < count; i += kWave) tab[__hip_atomic_load(&log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = 0u;
+        peak = count > peak ? count : peak;
         count = 0;
         __builtin_amdgcn_s_waitcnt(0);  // stores reach L2 before the next atomics
     }

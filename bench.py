@@ -45,6 +45,8 @@ WORKLOADS = {
                          "M=32, efC=256, 10k queries, k=10, CPU-built graph", n=1_000_000, nq=10_000),
     "gm100k_f32": dict(desc="100k x 128 float32 (rounded), 64-cluster gaussian mixture (seed 0), L2, M=16, "
                             "efC=200, 1k queries, k=10", n=100_000, nq=1_000),
+    "gm100m_u8": dict(desc="100M x 128 uint8, 1024-cluster gaussian mixture quantised to [0,255] (seed 4), L2, "
+                           "M=32, efC=256", n=100_000_000, nq=100_000, build="gpu"),
     "gm10m_f32": dict(desc="10M x 128 float32, 64-cluster gaussian mixture (seed 3), L2, M=32, efC=200, 100k "
                            "queries, k=10, GPU-built graph, fp16 traversal copy + FP32 re-rank",
                       n=10_000_000, nq=100_000, build="gpu", precision="f16"),
@@ -234,6 +236,117 @@ def summarize(args):
     print(f"wrote {args.out}")
 
 
+def run_serving(args):
+    """Open-loop serving run: Poisson arrivals of query batches (sizes
+    log-uniform in [1, 4096], ef uniform in [16, 256], k in {10, 100}) mixed
+    with insert batches of 1,024 new vectors at --insert-share of requests,
+    through the library's serving loop on a GPU-built index.  Reports p50 /
+    p99 request latency, completed queries per second and the loop's batching."""
+    import threading
+
+    from pickle_amd import abi, datasets
+
+    wl = WORKLOADS[args.workload]
+    n = args.n or wl["n"]
+    n_req = args.requests
+    n_ins = max(1, int(round(n_req * args.insert_share)))
+    extra = n_ins * 1024
+    base, queries, params = datasets.workload(args.workload, n=n + extra, nq=args.nq or 20000)
+    dim = base.shape[1]
+    mcode = abi.L2 if params["metric"] == "l2" else abi.IP
+    idx = abi.Index(base.dtype, mcode, dim, params["M"], params["efc"], n + extra)
+    levels = abi.sample_levels(n + extra, params["M"], args.seed)
+    idx.set_base(base[:n])
+    idx.init_build(levels)
+    tb = time.perf_counter()
+    idx.insert(np.arange(n, dtype=np.int32), max_batch=args.insert_batch)
+    build_s = time.perf_counter() - tb
+    log(f"serving index: {n} x {dim} {base.dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved)")
+
+    # capacity probe: one slot-sized launch at the mean ef
+    d_q = abi.DeviceBuffer.from_array(queries[:4096])
+    d_i, d_d, d_c = abi.DeviceBuffer(4096 * 100 * 4), abi.DeviceBuffer(4096 * 100 * 4), abi.DeviceBuffer(4096 * 4)
+    for _ in range(2):
+        idx.timer_begin()
+        idx.search_device(d_q, 4096, 10, 136, 10, d_i, d_d, d_c, None)
+        cap_ms = idx.timer_end()
+    cap_qps = 4096 / (cap_ms / 1e3)
+    mean_size = 4095 / np.log(4096)
+    rate = args.load * cap_qps / mean_size
+    log(f"capacity ~{cap_qps:,.0f} QPS at ef=136; offered load {args.load:.2f} -> {rate:,.1f} requests/s")
+
+    rng = np.random.default_rng(args.seed)
+    kinds = rng.permutation(np.r_[np.ones(n_ins, bool), np.zeros(n_req - n_ins, bool)])
+    gaps = rng.exponential(1.0 / rate, n_req)
+    sizes = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4096), n_req)))).astype(int)
+    efs = rng.integers(16, 257, n_req)
+    ks = rng.choice([10, 100], n_req)
+    efs = np.maximum(efs, ks)
+    srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=256)
+    tickets = []
+    done = []
+    t0 = time.perf_counter()
+
+    def producer():
+        t_next = t0
+        q_at, ins_at = 0, n
+        for i in range(n_req):
+            t_next += gaps[i]
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            if kinds[i]:
+                ids = np.arange(ins_at, ins_at + 1024, dtype=np.int32)
+                tickets.append(("insert", 1024, srv.insert(ids, base[ins_at:ins_at + 1024])))
+                ins_at += 1024
+            else:
+                m = int(sizes[i])
+                s = q_at % (len(queries) - m)
+                tickets.append(("search", m, srv.submit(queries[s:s + m], int(ks[i]), int(efs[i]))))
+                q_at += m
+        tickets.append(None)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    j = 0
+    while True:
+        while j >= len(tickets):
+            time.sleep(0.0005)
+        t = tickets[j]
+        j += 1
+        if t is None:
+            break
+        kind, m, tk = t
+        _, _, _, lat, que = srv.wait(tk)
+        done.append((kind, m, lat, que))
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat_s = np.array([d[2] for d in done if d[0] == "search"]) / 1e3
+    lat_i = np.array([d[2] for d in done if d[0] == "insert"]) / 1e3
+    nq_done = sum(d[1] for d in done if d[0] == "search")
+    line = {
+        "metric": "serving: p50 / p99 request latency and completed QPS under open-loop Poisson arrivals",
+        "value": round(nq_done / wall, 1), "unit": "queries/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
+        "ms_per_step": round(wall / n_req * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": str(base.dtype), "data": "synthetic",
+        "config": {"workload": f"{wl['desc']} (n={n}, GPU-built, +{extra} streamed)", "requests": n_req,
+                   "inserts": n_ins, "insert_batch": 1024, "slots": args.slots, "offered_load": args.load,
+                   "arrival_rate_per_s": round(rate, 1), "request_sizes": "log-uniform [1, 4096]",
+                   "ef": "uniform [16, 256]", "k": "{10, 100}",
+                   "p50_request_latency_ms": round(float(np.percentile(lat_s, 50)), 3),
+                   "p99_request_latency_ms": round(float(np.percentile(lat_s, 99)), 3),
+                   "p50_insert_latency_ms": round(float(np.percentile(lat_i, 50)), 3) if len(lat_i) else None,
+                   "queries": int(nq_done), "wall_s": round(wall, 3), "loop": st, "build_seconds": round(build_s, 2)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    log(f"served {nq_done} queries in {n_req - n_ins} requests + {n_ins} inserts in {wall:.2f}s: "
+        f"{nq_done / wall:,.0f} QPS, request latency p50 {line['config']['p50_request_latency_ms']} ms, "
+        f"p99 {line['config']['p99_request_latency_ms']} ms; loop {st}")
+    print(json.dumps(line), flush=True)
+
+
 def recall_at_k(ids, gt, k):
     """benchmark/hnsw_pickle.cpp:94-112: a returned id counts if it is one of
     the first k ground-truth labels (ties are not credited)."""
@@ -266,6 +379,17 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", choices=["none", "f16", "i8"], default=None,
                     help="search a low-precision traversal copy + FP32 re-rank (float workloads)")
+    ap.add_argument("--base-file", default=None, help=".npy / .fbin / .u8bin / .i8bin base vectors instead of "
+                                                      "a synthetic workload (with --query-file)")
+    ap.add_argument("--query-file", default=None)
+    ap.add_argument("--metric", choices=["l2", "ip"], default="l2", help="with --base-file")
+    ap.add_argument("--M", type=int, default=32, help="with --base-file")
+    ap.add_argument("--efc", type=int, default=200, help="with --base-file")
+    ap.add_argument("--serve", action="store_true", help="open-loop serving run through the request loop")
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--insert-share", type=float, default=0.05)
+    ap.add_argument("--load", type=float, default=0.6, help="offered load as a fraction of measured capacity")
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",
                     help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")
     ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -277,6 +401,12 @@ def main():
 
     if args.summarize_trace or args.summarize_pmc:
         summarize(args)
+        return
+
+    if args.serve:
+        if args.gpus != 1:
+            sys.exit("bench.py measures one GPU only")
+        run_serving(args)
         return
 
     if args.gpus != 1:
@@ -291,7 +421,18 @@ def main():
     nq = args.nq or wl["nq"]
     k = args.k
     t0 = time.time()
-    base, queries, params = datasets.workload(args.workload, n=n, nq=nq)
+    if args.base_file:
+        from pickle_amd import formats
+
+        base = np.ascontiguousarray(formats.read_vectors(args.base_file, 0, args.n))
+        queries = np.ascontiguousarray(formats.read_vectors(args.query_file, 0, args.nq))
+        n, nq = base.shape[0], queries.shape[0]
+        params = {"metric": args.metric, "M": args.M, "efc": args.efc}
+        wl = dict(wl, desc=f"{os.path.basename(args.base_file)} ({n} x {base.shape[1]} {base.dtype}), "
+                           f"{nq} queries from {os.path.basename(args.query_file)}, {args.metric}, M={args.M}, "
+                           f"efC={args.efc}", n=n, nq=nq)
+    else:
+        base, queries, params = datasets.workload(args.workload, n=n, nq=nq)
     dtype_name = {np.dtype(np.uint8): "u8", np.dtype(np.int8): "i8", np.dtype(np.float32): "f32",
                   np.dtype(np.float16): "f16"}[base.dtype]
     dim = base.shape[1]
@@ -312,7 +453,7 @@ def main():
 
     work = args.graph_cache or tempfile.mkdtemp(prefix="pk_bench_")
     os.makedirs(work, exist_ok=True)
-    tag = f"{args.workload}_n{n}_{build}"
+    tag = f"{os.path.basename(args.base_file) if args.base_file else args.workload}_n{n}_{build}"
     base_path = os.path.join(work, f"{tag}.base")
     q_path = os.path.join(work, f"{tag}_q{nq}.queries")
     graph_path = os.path.join(work, f"{tag}.graph")
@@ -335,7 +476,8 @@ def main():
     else:
         log(f"GPU batched build (insert batches up to {args.insert_batch}) ...")
         tb = time.perf_counter()
-        idx.build(base, max_batch=args.insert_batch, seed=args.seed)
+        idx.build(base, max_batch=args.insert_batch, seed=args.seed,
+                  progress=lambda d, t: log(f"  inserted {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
         idx.sync()
         build_info = {"build_seconds": time.perf_counter() - tb, "threads": 0, "where": "gpu",
                       "insert_batch": args.insert_batch}
@@ -347,14 +489,18 @@ def main():
 
     gt_nq = min(nq, args.gt_queries) if args.gt_queries else nq
     t1 = time.time()
-    gt_ids, gt_d = idx.brute_force(queries[:gt_nq])
+    parts = []
+    for s0 in range(0, gt_nq, 2000):
+        parts.append(idx.brute_force(queries[s0:min(gt_nq, s0 + 2000)]))
+        if n >= 5_000_000:
+            log(f"  ground truth {min(gt_nq, s0 + 2000):,} / {gt_nq:,} ({time.time() - t1:.1f}s)")
+    gt_ids = np.concatenate([p_[0] for p_ in parts])
     if base.dtype.kind == "f":  # FP64 re-rank of the FP32 candidates
-        A, Q = base.astype(np.float64), queries[:gt_nq].astype(np.float64)
         for i in range(gt_nq):
             c = gt_ids[i]
-            d64 = ((A[c] - Q[i]) ** 2).sum(1) if metric == "l2" else -(A[c] @ Q[i])
+            A, Q = base[c].astype(np.float64), queries[i].astype(np.float64)
+            d64 = ((A - Q) ** 2).sum(1) if metric == "l2" else -(A @ Q)
             gt_ids[i] = c[np.lexsort((c, d64))]
-        del A
     log(f"ground truth (GPU brute force, {gt_nq} queries) in {time.time() - t1:.1f}s")
 
     # HBM-resident buffers for the timed runs

@@ -46,6 +46,8 @@ extern "C" {
 #define PK_MAX_LEVELS 16
 
 typedef struct pk_index pk_index;
+typedef struct pk_search_graph pk_search_graph;
+typedef struct pk_server pk_server;
 
 /* Per-query work counters with the reference Profiler's definitions
  * (include/profiler.hpp:13-69, updated at include/hnsw.hpp:438-442 and
@@ -144,6 +146,59 @@ int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const in
                         const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max,
                         int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
                         int32_t *d_out_count, pk_query_counters *d_counters);
+
+/* One pk_search_device_ex launch captured into a HIP graph (no counterpart
+ * in the reference): the same buffers, shapes and per-query k / ef arrays
+ * are replayed by pk_search_graph_launch on the index's stream, so a serving
+ * loop or a repeated batch pays one graph launch.  Buffer contents may change
+ * between replays; pointers may not.  While a graph exists, calls that would
+ * reallocate what it points to (a search needing more scratch, loading
+ * layers, init_build, copy_graph) fail with PK_ERR_STATE. */
+int pk_search_graph_create(pk_index *h, const void *d_queries, int64_t nq, const int32_t *d_k,
+                           const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max,
+                           int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
+                           int32_t *d_out_count, pk_query_counters *d_counters,
+                           pk_search_graph **out);
+int pk_search_graph_launch(pk_search_graph *g);
+int pk_search_graph_destroy(pk_search_graph *g);
+
+/* Autotuner (no counterpart in the reference): times the search on the
+ * caller's sample queries (device buffer) at one (k, ef) over the resident
+ * workgroups per CU, the candidate heap's LDS depth and the visited-table
+ * size, keeps the fastest as the index's tuning, and returns it as
+ * out_tuning[3] = {ccap, vbits, blocks_per_cu} (0 = automatic) with its time
+ * per launch in *out_ms.  Results do not depend on the tuning. */
+int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k, int32_t ef,
+                       int32_t *out_tuning, float *out_ms);
+
+/* ---- serving loop (no counterpart in the reference; its callers run
+ * AnnSearch per request on host threads, include/hnsw.hpp:509-541) ----
+ * A request loop over one resident index with continuous batching: `slots`
+ * launch slots (1..8), each with its own stream, search scratch and pinned
+ * staging for `slot_queries` queries; a free slot takes waiting requests in
+ * arrival order (requests are not split), so requests that arrive while
+ * others run start as slots free up, and slots overlap on the device.  Each
+ * request has its own k <= max_k and k <= ef <= max_ef.  Inserts queue in
+ * the same order, run alone when no launch is in flight, and are visible to
+ * every search admitted after them (the index must come from
+ * pk_index_init_build).  All calls are thread-safe.  While a server exists
+ * the index's layers cannot be replaced. */
+int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
+/* Queues nq queries (host rows, copied at once); results go to the caller's
+ * out_ids / out_dists (nq x k) and out_count (nq, may be NULL) when the
+ * request completes. */
+int pk_server_submit(pk_server *s, const void *queries, int64_t nq, int32_t k, int32_t ef,
+                     int32_t *out_ids, float *out_dists, int32_t *out_count, int64_t *ticket);
+/* Queues Insert of n rows (host, by external id; copied at once). */
+int pk_server_insert(pk_server *s, const int32_t *ext_ids, const void *rows, int64_t n, int flags,
+                     int64_t *ticket);
+/* Waits for a request (timeout_ms < 0: no limit) and releases its ticket;
+ * latency_us = submit to completion, queue_us = submit to admission. */
+int pk_server_wait(pk_server *s, int64_t ticket, int32_t timeout_ms, double *latency_us, double *queue_us);
+/* out[5] = {launches, queries done, inserts done, most slots busy at once, requests waiting}. */
+int pk_server_stats(pk_server *s, int64_t *out);
+/* Finishes queued work, then frees the slots. */
+int pk_server_destroy(pk_server *s);
 
 /* Device failure flags of the last search, or'ed over its queries (0 = none;
  * 1 = a visited table filled up, 4 = a candidate heap filled up -- both

@@ -30,7 +30,9 @@ EXPORTS = [
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
     "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
-    "pk_index_copy_graph", "pk_search_rerank_device", "pk_search_rerank",
+    "pk_index_copy_graph", "pk_search_rerank_device", "pk_search_rerank", "pk_search_graph_create",
+    "pk_search_graph_launch", "pk_search_graph_destroy", "pk_autotune_search", "pk_server_create",
+    "pk_server_submit", "pk_server_insert", "pk_server_wait", "pk_server_stats", "pk_server_destroy",
 ]
 BUILD_EXACT_ROWS = 1
 BRUTE_K = 32
@@ -91,6 +93,17 @@ def lib():
             "pk_index_copy_graph": (C.c_int, [vp, vp]),
             "pk_search_rerank_device": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_rerank": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_graph_create": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp,
+                                                 C.POINTER(vp)]),
+            "pk_search_graph_launch": (C.c_int, [vp]),
+            "pk_search_graph_destroy": (C.c_int, [vp]),
+            "pk_autotune_search": (C.c_int, [vp, vp, i64, i32, i32, vp, C.POINTER(C.c_float)]),
+            "pk_server_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+            "pk_server_submit": (C.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, C.POINTER(i64)]),
+            "pk_server_insert": (C.c_int, [vp, vp, vp, i64, C.c_int, C.POINTER(i64)]),
+            "pk_server_wait": (C.c_int, [vp, i64, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "pk_server_stats": (C.c_int, [vp, vp]),
+            "pk_server_destroy": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -159,6 +172,88 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (pk_server_*): submit from any thread, wait per ticket.
+    ctypes releases the GIL during the calls, so Python threads submit and
+    wait concurrently."""
+
+    def __init__(self, index: "Index", slots: int = 4, slot_queries: int = 4096, max_k: int = 100,
+                 max_ef: int = 256):
+        self.index = index
+        self.max_k = max_k
+        s = C.c_void_p()
+        check(lib().pk_server_create(index.h, slots, slot_queries, max_k, max_ef, C.byref(s)))
+        self.s = s
+        self._keep = {}
+
+    def submit(self, queries: np.ndarray, k: int, ef: int) -> int:
+        q = np.ascontiguousarray(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        ds = np.empty((nq, k), np.float32)
+        cnt = np.empty(nq, np.int32)
+        t = C.c_int64()
+        check(lib().pk_server_submit(self.s, _p(q), nq, k, ef, _p(ids), _p(ds), _p(cnt), C.byref(t)))
+        self._keep[t.value] = (ids, ds, cnt)
+        return t.value
+
+    def insert(self, ext_ids: np.ndarray, rows: np.ndarray, exact_rows: bool = True) -> int:
+        e = np.ascontiguousarray(ext_ids, np.int32)
+        r = np.ascontiguousarray(rows)
+        t = C.c_int64()
+        check(lib().pk_server_insert(self.s, _p(e), _p(r), e.shape[0], BUILD_EXACT_ROWS if exact_rows else 0,
+                                     C.byref(t)))
+        self._keep[t.value] = None
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(ids, dists, count, latency_us, queue_us); ids etc. are None for inserts."""
+        lat, que = C.c_double(), C.c_double()
+        check(lib().pk_server_wait(self.s, ticket, timeout_ms, C.byref(lat), C.byref(que)))
+        res = self._keep.pop(ticket)
+        if res is None:
+            return None, None, None, lat.value, que.value
+        return res[0], res[1], res[2], lat.value, que.value
+
+    def stats(self) -> dict:
+        out = np.zeros(5, np.int64)
+        check(lib().pk_server_stats(self.s, _p(out)))
+        return dict(zip(["launches", "queries", "inserts", "max_concurrent", "waiting"], out.tolist()))
+
+    def close(self) -> None:
+        if self.s:
+            check(lib().pk_server_destroy(self.s))
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SearchGraph:
+    """A captured search launch (pk_search_graph_*)."""
+
+    def __init__(self, g, index):
+        self.g = g
+        self.index = index  # keeps the index alive while the graph exists
+
+    def launch(self) -> None:
+        check(lib().pk_search_graph_launch(self.g))
+
+    def close(self) -> None:
+        if self.g:
+            check(lib().pk_search_graph_destroy(self.g))
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
@@ -289,15 +384,22 @@ class Index:
                                     BUILD_EXACT_ROWS if exact_rows else 0))
 
     def build(self, base: np.ndarray, levels: np.ndarray | None = None, order=None, max_batch: int = 4096,
-              exact_rows: bool = True, seed: int = 0) -> None:
-        """Build (include/hnsw.hpp:497-507): base rows, levels, inserts."""
+              exact_rows: bool = True, seed: int = 0, progress=None, chunk: int = 1 << 20) -> None:
+        """Build (include/hnsw.hpp:497-507): base rows, levels, inserts.  With
+        `progress`, ids are inserted in chunks and progress(done, total) is
+        called after each."""
         self.set_base(base)
         if levels is None:
             levels = sample_levels(self.capacity, self.M, seed)
         self.init_build(levels)
         if order is None:
             order = np.arange(base.shape[0], dtype=np.int32)
-        self.insert(order, max_batch=max_batch, exact_rows=exact_rows)
+        if progress is None:
+            self.insert(order, max_batch=max_batch, exact_rows=exact_rows)
+            return
+        for s in range(0, len(order), chunk):
+            self.insert(order[s:s + chunk], max_batch=max_batch, exact_rows=exact_rows)
+            progress(min(len(order), s + chunk), len(order))
 
     def brute_force(self, queries: np.ndarray):
         """The BRUTE_K nearest base rows per query (exact for u8/i8; FP32 for floats)."""
@@ -315,6 +417,29 @@ class Index:
         f = C.c_int()
         check(lib().pk_device_error(self.h, C.byref(f)))
         return f.value
+
+    # ---- HIP graph of one search launch, autotuner ----
+    def search_graph(self, d_queries: "DeviceBuffer", nq: int, k, ef, k_stride: int, d_ids: "DeviceBuffer",
+                     d_dists: "DeviceBuffer", d_count: "DeviceBuffer" = None, d_counters: "DeviceBuffer" = None,
+                     ef_max: int | None = None) -> "SearchGraph":
+        """k / ef: ints (one for all queries) or DeviceBuffers of int32 per query
+        (then ef_max is required)."""
+        dk = k.ptr if isinstance(k, DeviceBuffer) else None
+        de = ef.ptr if isinstance(ef, DeviceBuffer) else None
+        k_all = 0 if dk else int(k)
+        ef_all = 0 if de else int(ef)
+        em = ef_max if de else ef_all
+        g = C.c_void_p()
+        check(lib().pk_search_graph_create(self.h, d_queries.ptr, nq, dk, de, k_all, ef_all, em, k_stride, d_ids.ptr,
+                                           d_dists.ptr, d_count.ptr if d_count else None,
+                                           d_counters.ptr if d_counters else None, C.byref(g)))
+        return SearchGraph(g, self)
+
+    def autotune(self, d_queries: "DeviceBuffer", nq: int, k: int, ef: int):
+        t = np.zeros(3, np.int32)
+        ms = C.c_float()
+        check(lib().pk_autotune_search(self.h, d_queries.ptr, nq, k, ef, _p(t), C.byref(ms)))
+        return {"ccap": int(t[0]), "vbits": int(t[1]), "blocks_per_cu": int(t[2]), "ms": ms.value}
 
     # ---- precision form ----
     def set_quantizer(self, offset: np.ndarray, scale: np.ndarray) -> None:

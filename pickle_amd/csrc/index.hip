@@ -231,6 +231,7 @@ int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capa
         delete h;
         return fail(PK_ERR_HIP, "stream / event creation failed");
     }
+    h->sc.stream = h->stream;
     int dev = 0;
     (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -241,13 +242,12 @@ int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capa
 }
 
 int pk_index_destroy(pk_index* h) {
+    if (h && h->graphs_alive) return fail(PK_ERR_STATE, "destroy the index's search graphs first");
     if (!h) return PK_OK;
     (void)hipStreamSynchronize(h->stream);
     for (auto& L : h->layers) free_layer(L);
     dfree(h->base);
-    dfree(h->cglob);
-    dfree(h->vtab);
-    dfree(h->vlog);
+    free_search_ctx(h->sc);
     if (h->d_layers) (void)hipFree(h->d_layers);
     if (h->d_build) (void)hipFree(h->d_build);
     dfree(h->ins.d_ext);
@@ -261,8 +261,6 @@ int pk_index_destroy(pk_index* h) {
     dfree(h->ins.vlog);
     dfree(h->ins.counter);
     dfree(h->ins.err);
-    dfree(h->qcounter);
-    dfree(h->err);
     for (auto& b : h->io) dfree(b.p);
     for (auto& b : h->rr) dfree(b.p);
     dfree(h->full);
@@ -316,6 +314,7 @@ int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_le
                         const int32_t* degree, const int16_t* heur, const int32_t* adj, const float* dist,
                         const uint8_t* has_vector) {
     if (!h || !ext || !degree || !adj || !has_vector) return fail(PK_ERR_ARG, "null argument");
+    if (h->graphs_alive) return fail(PK_ERR_STATE, "destroy the search graphs before replacing layers");
     if (level < 0 || level >= PK_MAX_LEVELS) return fail(PK_ERR_ARG, "level %d out of range", level);
     if (n < 0 || n > h->capacity || D < 1 || adj_len < n * D) return fail(PK_ERR_ARG, "bad layer shape");
     if (!h->base) return fail(PK_ERR_STATE, "set the base rows before loading layers");
@@ -419,32 +418,61 @@ int pk_index_read_layer(pk_index* h, int level, int32_t* ext, int32_t* degree, i
 // search
 // ---------------------------------------------------------------------------
 
-static int ensure_scratch(pk_index* h, int waves, int vbits) {
-    if (waves <= h->scratch_waves && vbits <= h->scratch_vbits) return PK_OK;
-    waves = std::max(waves, h->scratch_waves);
-    vbits = std::max(vbits, h->scratch_vbits);
-    HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(h->cglob);
-    dfree(h->vtab);
-    dfree(h->vlog);
-    h->scratch_waves = 0;
-    h->scratch_vbits = 0;
-    int rc;
-    size_t slots = (size_t)waves << vbits;
-    if ((rc = dalloc(&h->cglob, (size_t)waves * h->cg_cap))) return rc;
-    if ((rc = dalloc(&h->vtab, slots))) return rc;
-    if ((rc = dalloc(&h->vlog, slots))) return rc;
-    HIPCHK(hipMemsetAsync(h->vtab, 0, slots * sizeof(uint32_t), h->stream));
-    if (!h->qcounter && (rc = dalloc(&h->qcounter, 1))) return rc;
-    if (!h->err && (rc = dalloc(&h->err, 1))) return rc;
-    h->scratch_waves = waves;
-    h->scratch_vbits = vbits;
-    return PK_OK;
+}  // extern "C"
+
+namespace pk {
+void free_search_ctx(SearchCtx& c) {
+    dfree(c.cglob);
+    dfree(c.vtab);
+    dfree(c.vlog);
+    dfree(c.qcounter);
+    dfree(c.err);
+    c.waves = c.vbits = 0;
 }
 
-int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
-                        int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
-                        float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+static int ensure_scratch(pk_index* h, SearchCtx& c, int waves, int vbits) {
+    if (waves <= c.waves && vbits <= c.vbits) return PK_OK;
+    if (&c == &h->sc && h->graphs_alive)
+        return fail(PK_ERR_STATE, "this search needs more scratch than the captured search graphs use; destroy them "
+                                  "first");
+    waves = std::max(waves, c.waves);
+    vbits = std::max(vbits, c.vbits);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    dfree(c.cglob);
+    dfree(c.vtab);
+    dfree(c.vlog);
+    c.waves = 0;
+    c.vbits = 0;
+    int rc;
+    size_t slots = (size_t)waves << vbits;
+    if ((rc = dalloc(&c.cglob, (size_t)waves * h->cg_cap))) return rc;
+    if ((rc = dalloc(&c.vtab, slots))) return rc;
+    if ((rc = dalloc(&c.vlog, slots))) return rc;
+    HIPCHK(hipMemsetAsync(c.vtab, 0, slots * sizeof(uint32_t), c.stream));
+    if (!c.qcounter && (rc = dalloc(&c.qcounter, 1))) return rc;
+    if (!c.err && (rc = dalloc(&c.err, 1))) return rc;
+    c.waves = waves;
+    c.vbits = vbits;
+    return PK_OK;
+}
+}  // namespace pk
+
+extern "C" {
+
+}  // extern "C"
+
+namespace pk {
+struct SearchPlan {
+    SearchArgs a;
+    int grid = 0;
+    size_t lds = 0;
+};
+
+// validation, LDS plan and scratch (the part that may allocate)
+static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
+                       const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
+                       int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
+                       SearchPlan* P) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
     if (!h->ready) return fail(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
     if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return fail(PK_ERR_ARG, "bad buffers");
@@ -452,10 +480,10 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     if (!d_k && (k_all < 1 || k_all > k_stride)) return fail(PK_ERR_ARG, "k must be in [1, k_stride]");
     if (!d_ef && ef_all < 1) return fail(PK_ERR_ARG, "ef must be positive");
     if (ef_max < 1) return fail(PK_ERR_ARG, "ef_max must be positive");
-    if (nq == 0) return PK_OK;
     if (nq > (int64_t)1 << 27) return fail(PK_ERR_CAPACITY, "at most 2^27 queries per call");
 
-    SearchArgs a{};
+    SearchArgs& a = P->a;
+    a = SearchArgs{};
     a.layers = static_cast<const LayerDev*>(h->d_layers);
     a.ext0 = h->layers[0].ext;
     a.enter_level = h->enter_level;
@@ -475,8 +503,9 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     a.ctr = d_counters;
 
     // LDS plan per wave: result heap, candidate heap (deeper levels spill to
-    // device memory), candidate id / distance lists, query row.  The visited table lives in device memory,
-    // sized for the visits ef implies (a full table fails the query).
+    // device memory), candidate id / distance lists, query row.  The visited
+    // table lives in device memory, sized for the visits ef implies (a full
+    // table fails the query).
     const SearchTuning& t = h->tuning;
     int wcap = result_capacity_for(ef_max);
     if (wcap < 0) return fail(PK_ERR_ARG, "ef_max %d above the largest supported ef (630)", ef_max);
@@ -490,25 +519,189 @@ int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const in
     int bpc = 0;
     if (search_occupancy(h->dtype, h->metric, h->size_class, lds, &bpc) != hipSuccess || bpc < 1) bpc = 1;
     if (t.blocks_per_cu > 0) bpc = std::min(bpc, t.blocks_per_cu);
-    int64_t want = (nq + (kSearchBlock / 64) - 1) / (kSearchBlock / 64);
+    int64_t want = std::max<int64_t>(1, (nq + (kSearchBlock / 64) - 1) / (kSearchBlock / 64));
     int grid = (int)std::min<int64_t>(want, (int64_t)h->num_cus * bpc);
     int waves = grid * (kSearchBlock / 64);
-    int rc = ensure_scratch(h, waves, vbits);
+    int rc = ensure_scratch(h, c, waves, vbits);
     if (rc) return rc;
-    a.cglob = h->cglob;
-    a.vtab = h->vtab;
-    a.vlog = h->vlog;
+    a.cglob = c.cglob;
+    a.vtab = c.vtab;
+    a.vlog = c.vlog;
     a.cg_cap = h->cg_cap;
-    a.vg_bits = h->scratch_vbits;
+    a.vg_bits = vbits;
     a.wcap = wcap;
     a.ccap = ccap;
     a.lds_per_wave = (int)per_wave;
-    a.qcounter = h->qcounter;
-    a.err = h->err;
-    HIPCHK(hipMemsetAsync(h->err, 0, sizeof(int32_t), h->stream));
-    h->last_grid = grid;
-    h->last_lds = lds;
-    HIPCHK(launch_search(a, h->dtype, h->metric, h->size_class, grid, lds, h->stream));
+    a.qcounter = c.qcounter;
+    a.err = c.err;
+    P->grid = grid;
+    P->lds = lds;
+    return PK_OK;
+}
+
+// the kernels only (capturable)
+static int enqueue_search(pk_index* h, SearchCtx& c, const SearchPlan& P) {
+    if (&c == &h->sc) {
+        h->last_grid = P.grid;
+        h->last_lds = P.lds;
+    }
+    HIPCHK(launch_search(P.a, h->dtype, h->metric, h->size_class, P.grid, P.lds, c.stream));
+    return PK_OK;
+}
+
+int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+              int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
+              int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (h && nq == 0 && d_out_ids) return PK_OK;
+    SearchPlan P;
+    int rc = plan_search(h, c, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids, d_out_dists,
+                         d_out_count, d_counters, &P);
+    if (rc) return rc;
+    if (nq == 0) return PK_OK;
+    return enqueue_search(h, c, P);
+}
+}  // namespace pk
+
+extern "C" {
+
+int pk_search_device_ex(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                        int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                        float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    return search_on(h, h->sc, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids, d_out_dists,
+                     d_out_count, d_counters);
+}
+
+// ---- HIP graphs of one search launch ----
+
+int pk_search_graph_create(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                           int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                           float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
+                           pk_search_graph** out) {
+    if (!out) return fail(PK_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (nq < 1) return fail(PK_ERR_ARG, "a search graph needs at least one query");
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    SearchPlan P;
+    int rc = plan_search(h, h->sc, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids, d_out_dists,
+                         d_out_count, d_counters, &P);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_search(h, h->sc, P);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(PK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(PK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    auto* sg = new pk_search_graph{h, g, ex, P.grid};
+    h->graphs_alive++;
+    *out = sg;
+    return PK_OK;
+}
+
+int pk_search_graph_launch(pk_search_graph* g) {
+    if (!g) return fail(PK_ERR_ARG, "null graph");
+    HIPCHK(hipGraphLaunch(g->exec, g->h->stream));
+    g->h->last_grid = g->grid;
+    return PK_OK;
+}
+
+int pk_search_graph_destroy(pk_search_graph* g) {
+    if (!g) return PK_OK;
+    (void)hipStreamSynchronize(g->h->stream);
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    g->h->graphs_alive--;
+    delete g;
+    return PK_OK;
+}
+
+// ---- autotuner ----
+
+int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k, int32_t ef, int32_t* out_tuning,
+                       float* out_ms) {
+    if (!h || !d_queries || nq < 1 || k < 1 || ef < k) return fail(PK_ERR_ARG, "bad arguments");
+    if (h->graphs_alive) return fail(PK_ERR_STATE, "destroy the search graphs before tuning");
+    int rc;
+    uint8_t* buf = nullptr;
+    const size_t out_bytes = (size_t)nq * k * 8 + (size_t)nq * 4;
+    if ((rc = dalloc(&buf, out_bytes))) return rc;
+    int32_t* ids = reinterpret_cast<int32_t*>(buf);
+    float* ds = reinterpret_cast<float*>(buf + (size_t)nq * k * 4);
+    int32_t* cnt = reinterpret_cast<int32_t*>(buf + (size_t)nq * k * 8);
+    const SearchTuning saved = h->tuning;
+    auto time_one = [&](const SearchTuning& t, float* ms) -> int {
+        h->tuning = t;
+        int r = pk_search_device_ex(h, d_queries, nq, nullptr, nullptr, k, ef, ef, k, ids, ds, cnt, nullptr);
+        if (r) return r;
+        HIPCHK(hipEventRecord(h->ev0, h->stream));
+        for (int i = 0; i < 2 && !r; i++)
+            r = pk_search_device_ex(h, d_queries, nq, nullptr, nullptr, k, ef, ef, k, ids, ds, cnt, nullptr);
+        if (r) return r;
+        HIPCHK(hipEventRecord(h->ev1, h->stream));
+        HIPCHK(hipEventSynchronize(h->ev1));
+        HIPCHK(hipEventElapsedTime(ms, h->ev0, h->ev1));
+        *ms /= 2;
+        return PK_OK;
+    };
+    // coordinate descent: resident workgroups per CU, then the candidate
+    // heap's LDS depth, then the visited-table size (configurations that do
+    // not fit are skipped)
+    SearchTuning best = saved;
+    float best_ms = 0;
+    if ((rc = time_one(best, &best_ms))) {
+        h->tuning = saved;
+        dfree(buf);
+        return rc;
+    }
+    auto try_set = [&](SearchTuning t) {
+        float ms = 0;
+        if (time_one(t, &ms) == PK_OK && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    int max_bpc = 0;
+    {
+        SearchPlan P;
+        h->tuning = best;
+        if ((rc = plan_search(h, h->sc, d_queries, nq, nullptr, nullptr, k, ef, ef, k, ids, ds, cnt, nullptr, &P)) ==
+            PK_OK)
+            max_bpc = (P.grid + h->num_cus - 1) / h->num_cus;
+    }
+    for (int b = max_bpc - 1; b >= std::max(1, max_bpc - 3); b--) {
+        SearchTuning t = best;
+        t.blocks_per_cu = b;
+        try_set(t);
+    }
+    for (int c : {256, 512, 1024, 2048}) {
+        SearchTuning t = best;
+        t.ccap = c;
+        try_set(t);
+    }
+    const int vb0 = ef <= 32 ? 14 : (ef <= 128 ? 15 : (ef <= 512 ? 16 : 17));
+    for (int v : {vb0 - 1, vb0 + 1}) {
+        SearchTuning t = best;
+        t.vbits = v;
+        try_set(t);
+    }
+    h->tuning = best;
+    dfree(buf);
+    if (out_tuning) {
+        out_tuning[0] = best.ccap;
+        out_tuning[1] = best.vbits;
+        out_tuning[2] = best.blocks_per_cu;
+    }
+    if (out_ms) *out_ms = best_ms;
     return PK_OK;
 }
 
@@ -521,9 +714,9 @@ int pk_search_device(pk_index* h, const void* d_queries, int64_t nq, const int32
 }
 
 int pk_device_error(pk_index* h, int* flags) {
-    if (!h || !h->err) return fail(PK_ERR_STATE, "no search has run");
+    if (!h || !h->sc.err) return fail(PK_ERR_STATE, "no search has run");
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(flags, h->err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(flags, h->sc.err, sizeof(int32_t), hipMemcpyDeviceToHost));
     return PK_OK;
 }
 
@@ -583,7 +776,7 @@ int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, co
     if (counters)
         HIPCHK(hipMemcpyAsync(counters, dctr, (size_t)nq * sizeof(pk_query_counters), hipMemcpyDeviceToHost, h->stream));
     int32_t errflag = 0;
-    HIPCHK(hipMemcpyAsync(&errflag, h->err, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&errflag, h->sc.err, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (errflag) {
         int64_t bad = 0;

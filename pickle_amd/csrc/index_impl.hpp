@@ -54,6 +54,19 @@ struct SearchTuning {
     int blocks_per_cu = 0;  // cap on resident workgroups per CU (0 = occupancy)
 };
 
+// search scratch bound to one stream: per resident wave a candidate-heap
+// spill area, a visited table and its log; the work counter and failure flags
+struct SearchCtx {
+    hipStream_t stream = nullptr;
+    uint64_t* cglob = nullptr;
+    uint32_t* vtab = nullptr;
+    int32_t* vlog = nullptr;
+    int waves = 0;
+    int vbits = 0;
+    int32_t* qcounter = nullptr;
+    int32_t* err = nullptr;
+};
+
 struct IoBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -73,6 +86,12 @@ int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int wa
 int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
                         int maxl);
 int gather_layer_vectors(pk_index* h, int level);
+// one batched search on a given context (its stream and scratch); the
+// handle's own calls use h->sc
+int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+              int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
+              int32_t* d_out_count, pk_query_counters* d_counters);
+void free_search_ctx(SearchCtx& c);
 }  // namespace pk
 
 struct pk_index {
@@ -94,15 +113,9 @@ struct pk_index {
     pk::InsertScratch ins;
     int enter_level = 0;
     bool ready = false;
-    // search scratch (per resident wave)
-    uint64_t* cglob = nullptr;
-    uint32_t* vtab = nullptr;
-    int32_t* vlog = nullptr;
-    int scratch_waves = 0;
-    int scratch_vbits = 0;
+    // search scratch of the handle's own stream
+    pk::SearchCtx sc;
     int cg_cap = 16384;
-    int32_t* qcounter = nullptr;
-    int32_t* err = nullptr;
     pk::SearchTuning tuning;
     pk::IoBuf io[8];
     int last_grid = 0;
@@ -116,4 +129,12 @@ struct pk_index {
     float* q_inv_scale = nullptr;
     bool has_quantizer = false;
     pk::IoBuf rr[5];  // re-rank scratch: converted queries, candidate ids / dists / counts, counter
+    int graphs_alive = 0;  // captured search graphs hold pointers into scratch and layers
+};
+
+struct pk_search_graph {
+    pk_index* h;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int grid;
 };

@@ -297,6 +297,7 @@ int pk_index_copy_graph(pk_index* dst, pk_index* src) {
     if (dst->capacity != src->capacity || dst->dim != src->dim || dst->M != src->M)
         return set_error(PK_ERR_ARG, "indexes differ in capacity, dim or M");
     if (!dst->base) return set_error(PK_ERR_STATE, "set the destination's rows first");
+    if (dst->graphs_alive) return set_error(PK_ERR_STATE, "destroy the destination's search graphs first");
     PCHK(hipStreamSynchronize(src->stream));
     PCHK(hipStreamSynchronize(dst->stream));
     int rc;

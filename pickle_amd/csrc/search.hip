@@ -167,7 +167,10 @@ int result_capacity_for(int ef_max) {
 hipError_t launch_search(const SearchArgs& a, int dtype, int metric, int size_class, int grid, size_t lds,
                          hipStream_t s) {
     hipError_t e = hipSuccess;
+    // counters are cleared by kernels, not memset nodes, so the sequence
+    // can be captured into a HIP graph and replayed
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.err, 0, 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     switch (dtype * 4 + metric) {
         case DT_U8 * 4 + MET_L2: return launch_dt<DT_U8, MET_L2>(a, size_class, grid, lds, s);

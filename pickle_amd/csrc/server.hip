@@ -1,0 +1,390 @@
+// server.hip -- continuous-batching request loop over one resident index.
+//
+// Callers submit query batches of any size (up to a slot's capacity) with
+// their own k and ef from any host thread; the loop keeps the index in HBM
+// and runs several launch slots at once.  Each slot owns a HIP stream, the
+// search scratch for its resident waves (visited tables, candidate-heap
+// spill) and pinned staging for up to `slot_queries` queries: admission is
+// by that per-query scratch.  Whenever a slot is free, the dispatcher packs
+// waiting requests into it in arrival order (a request is never split), so
+// requests that arrive while others are on the GPU start as soon as a slot
+// frees, and slots overlap on the device.  One launch carries per-query k
+// and ef arrays, so requests with different k / ef share it.
+//
+// Inserts go through the same loop: an insert waits until no launch is in
+// flight, runs alone (the reference's Insert is not safe beside AnnSearch,
+// include/hnsw.hpp:247-302), and every search admitted after it sees the new
+// nodes.  A query's result depends only on the graph it ran on, so it is the
+// same alone and in any mix.
+//
+// All HIP work happens on the dispatcher thread.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "index_impl.hpp"
+
+using namespace pk;
+
+namespace {
+
+#define SCHK(expr)                                                                                        \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return set_error(PK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t ticket = 0;
+    std::vector<uint8_t> queries;  // nq rows (copied at submit)
+    int64_t nq = 0;
+    int k = 0, ef = 0;
+    int32_t* out_ids = nullptr;  // caller-owned, nq x k
+    float* out_dists = nullptr;
+    int32_t* out_count = nullptr;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    bool done = false;
+    int status = PK_OK;
+    // insert request
+    bool is_insert = false;
+    std::vector<int32_t> ext_ids;
+    int flags = 0;
+};
+
+struct Slot {
+    SearchCtx ctx;
+    uint8_t* d_q = nullptr;
+    int32_t *d_k = nullptr, *d_ef = nullptr, *d_ids = nullptr, *d_cnt = nullptr;
+    float* d_d = nullptr;
+    uint8_t* h_q = nullptr;  // pinned
+    int32_t *h_k = nullptr, *h_ef = nullptr, *h_ids = nullptr, *h_cnt = nullptr;
+    float* h_d = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<Request*> reqs;
+    std::vector<int64_t> first;  // first query row of each request in the launch
+    bool busy = false;
+    int64_t launched = 0;
+};
+
+}  // namespace
+
+struct pk_server {
+    pk_index* h = nullptr;
+    int slot_queries = 0, max_k = 0, max_ef = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<int64_t, Request*> by_ticket;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    int device = 0;
+    std::thread th;
+    // statistics
+    int64_t launches = 0, queries_done = 0, inserts_done = 0, max_concurrent = 0;
+};
+
+namespace {
+
+void free_slot(Slot& s) {
+    free_search_ctx(s.ctx);
+    for (void* p : {(void*)s.d_q, (void*)s.d_k, (void*)s.d_ef, (void*)s.d_ids, (void*)s.d_cnt, (void*)s.d_d})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)s.h_q, (void*)s.h_k, (void*)s.h_ef, (void*)s.h_ids, (void*)s.h_cnt, (void*)s.h_d})
+        if (p) (void)hipHostFree(p);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.ctx.stream) (void)hipStreamDestroy(s.ctx.stream);
+}
+
+int alloc_slot(pk_server* S, Slot& s) {
+    const int64_t q = S->slot_queries;
+    const size_t rb = (size_t)S->h->qbytes;
+    SCHK(hipStreamCreateWithFlags(&s.ctx.stream, hipStreamNonBlocking));
+    SCHK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_q), q * rb));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_k), q * 4));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_ef), q * 4));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_ids), q * S->max_k * 4));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_d), q * S->max_k * 4));
+    SCHK(hipMalloc(reinterpret_cast<void**>(&s.d_cnt), q * 4));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_q), q * rb, hipHostMallocDefault));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_k), q * 4, hipHostMallocDefault));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_ef), q * 4, hipHostMallocDefault));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_ids), q * S->max_k * 4, hipHostMallocDefault));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_d), q * S->max_k * 4, hipHostMallocDefault));
+    SCHK(hipHostMalloc(reinterpret_cast<void**>(&s.h_cnt), q * 4, hipHostMallocDefault));
+    return PK_OK;
+}
+
+// launch: copy in, search with per-query k / ef, copy out, record the event
+int launch_slot(pk_server* S, Slot& s) {
+    const size_t rb = (size_t)S->h->qbytes;
+    int64_t nq = 0;
+    s.first.clear();
+    for (Request* r : s.reqs) {
+        s.first.push_back(nq);
+        std::memcpy(s.h_q + nq * rb, r->queries.data(), r->nq * rb);
+        for (int64_t i = 0; i < r->nq; i++) {
+            s.h_k[nq + i] = r->k;
+            s.h_ef[nq + i] = r->ef;
+        }
+        nq += r->nq;
+    }
+    hipStream_t st = s.ctx.stream;
+    SCHK(hipMemcpyAsync(s.d_q, s.h_q, nq * rb, hipMemcpyHostToDevice, st));
+    SCHK(hipMemcpyAsync(s.d_k, s.h_k, nq * 4, hipMemcpyHostToDevice, st));
+    SCHK(hipMemcpyAsync(s.d_ef, s.h_ef, nq * 4, hipMemcpyHostToDevice, st));
+    int rc = search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, S->max_ef, S->max_k, s.d_ids, s.d_d, s.d_cnt,
+                       nullptr);
+    if (rc) return rc;
+    SCHK(hipMemcpyAsync(s.h_ids, s.d_ids, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
+    SCHK(hipMemcpyAsync(s.h_d, s.d_d, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
+    SCHK(hipMemcpyAsync(s.h_cnt, s.d_cnt, nq * 4, hipMemcpyDeviceToHost, st));
+    SCHK(hipEventRecord(s.done, st));
+    s.launched = nq;
+    return PK_OK;
+}
+
+// results of a finished slot into the requests' own buffers
+void complete_slot(pk_server* S, Slot& s, int status) {
+    const double t = now_us();
+    for (size_t j = 0; j < s.reqs.size(); j++) {
+        Request* r = s.reqs[j];
+        const int64_t f = s.first[j];
+        if (status == PK_OK) {
+            for (int64_t i = 0; i < r->nq; i++) {
+                std::memcpy(r->out_ids + i * r->k, s.h_ids + (f + i) * S->max_k, r->k * 4);
+                std::memcpy(r->out_dists + i * r->k, s.h_d + (f + i) * S->max_k, r->k * 4);
+            }
+            if (r->out_count) std::memcpy(r->out_count, s.h_cnt + f, r->nq * 4);
+        }
+        r->status = status;
+        r->t_done = t;
+        r->done = true;
+        S->queries_done += r->nq;
+    }
+    s.reqs.clear();
+    s.busy = false;
+}
+
+void dispatcher(pk_server* S) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (true) {
+        bool progress = false;
+        // finished launches
+        for (Slot& s : S->slots) {
+            if (!s.busy) continue;
+            hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            int status = PK_OK;
+            if (q != hipSuccess) status = set_error(PK_ERR_HIP, std::string("search launch: ") + hipGetErrorString(q));
+            complete_slot(S, s, status);
+            progress = true;
+        }
+        int busy = 0;
+        for (Slot& s : S->slots) busy += s.busy;
+        if (progress) S->cv_done.notify_all();
+        if (S->stop && S->pending.empty() && busy == 0) break;
+        // an insert at the head of the queue runs alone
+        if (!S->pending.empty() && S->pending.front()->is_insert) {
+            if (busy == 0) {
+                Request* r = S->pending.front();
+                S->pending.pop_front();
+                r->t_start = now_us();
+                lk.unlock();
+                // rows by external id (contiguous runs in one copy each), then Insert
+                int rc = PK_OK;
+                const int64_t n = (int64_t)r->ext_ids.size();
+                for (int64_t i = 0; i < n && rc == PK_OK;) {
+                    int64_t j = i + 1;
+                    while (j < n && r->ext_ids[j] == r->ext_ids[j - 1] + 1) j++;
+                    rc = pk_index_set_base(S->h, r->queries.data() + i * S->h->qbytes, r->ext_ids[i], j - i);
+                    i = j;
+                }
+                if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data(), n, 4096, r->flags);
+                lk.lock();
+                r->status = rc;
+                r->t_done = now_us();
+                r->done = true;
+                S->inserts_done++;
+                S->cv_done.notify_all();
+                continue;
+            }
+        } else {
+            // admission: free slots take waiting requests in arrival order
+            for (Slot& s : S->slots) {
+                if (s.busy || S->pending.empty() || S->pending.front()->is_insert) continue;
+                int64_t nq = 0;
+                while (!S->pending.empty() && !S->pending.front()->is_insert &&
+                       nq + S->pending.front()->nq <= S->slot_queries) {
+                    Request* r = S->pending.front();
+                    S->pending.pop_front();
+                    r->t_start = now_us();
+                    nq += r->nq;
+                    s.reqs.push_back(r);
+                }
+                if (s.reqs.empty()) continue;
+                s.busy = true;
+                S->launches++;
+                int rc = launch_slot(S, s);
+                if (rc) complete_slot(S, s, rc);
+                progress = true;
+            }
+            busy = 0;
+            for (Slot& s : S->slots) busy += s.busy;
+            S->max_concurrent = std::max<int64_t>(S->max_concurrent, busy);
+            if (progress) S->cv_done.notify_all();
+        }
+        if (progress) continue;
+        if (busy > 0) {
+            // launches in flight: poll their events without holding the lock
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        } else {
+            S->cv_work.wait_for(lk, std::chrono::milliseconds(50));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_server_create(pk_index* h, int slots, int slot_queries, int max_k, int max_ef, pk_server** out) {
+    if (!h || !out) return set_error(PK_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!h->ready) return set_error(PK_ERR_STATE, "index not loaded");
+    if (slots < 1 || slots > 8 || slot_queries < 1 || max_k < 1 || max_ef < max_k || max_ef > 630)
+        return set_error(PK_ERR_ARG, "need 1..8 slots, slot_queries >= 1, 1 <= max_k <= max_ef <= 630");
+    auto* S = new pk_server;
+    S->h = h;
+    S->slot_queries = slot_queries;
+    S->max_k = max_k;
+    S->max_ef = max_ef;
+    S->slots.resize(slots);
+    for (Slot& s : S->slots) {
+        int rc = alloc_slot(S, s);
+        if (rc) {
+            for (Slot& t : S->slots) free_slot(t);
+            delete S;
+            return rc;
+        }
+    }
+    (void)hipGetDevice(&S->device);
+    h->graphs_alive++;  // the slots' launches read the layers: no reallocation meanwhile
+    S->th = std::thread(dispatcher, S);
+    *out = S;
+    return PK_OK;
+}
+
+int pk_server_submit(pk_server* S, const void* queries, int64_t nq, int32_t k, int32_t ef, int32_t* out_ids,
+                     float* out_dists, int32_t* out_count, int64_t* ticket) {
+    if (!S || !queries || !out_ids || !out_dists || !ticket) return set_error(PK_ERR_ARG, "null argument");
+    if (nq < 1 || nq > S->slot_queries) return set_error(PK_ERR_CAPACITY, "request larger than a slot");
+    if (k < 1 || k > S->max_k || ef < k || ef > S->max_ef) return set_error(PK_ERR_ARG, "k / ef outside the server's limits");
+    auto* r = new Request;
+    r->queries.assign(static_cast<const uint8_t*>(queries), static_cast<const uint8_t*>(queries) + nq * S->h->qbytes);
+    r->nq = nq;
+    r->k = k;
+    r->ef = ef;
+    r->out_ids = out_ids;
+    r->out_dists = out_dists;
+    r->out_count = out_count;
+    r->t_submit = now_us();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        if (S->stop) {
+            delete r;
+            return set_error(PK_ERR_STATE, "server is stopping");
+        }
+        r->ticket = S->next_ticket++;
+        S->by_ticket[r->ticket] = r;
+        S->pending.push_back(r);
+        *ticket = r->ticket;
+    }
+    S->cv_work.notify_one();
+    return PK_OK;
+}
+
+int pk_server_insert(pk_server* S, const int32_t* ext_ids, const void* rows, int64_t n, int flags, int64_t* ticket) {
+    if (!S || !ext_ids || !rows || !ticket || n < 1) return set_error(PK_ERR_ARG, "bad arguments");
+    auto* r = new Request;
+    r->is_insert = true;
+    r->ext_ids.assign(ext_ids, ext_ids + n);
+    r->queries.assign(static_cast<const uint8_t*>(rows), static_cast<const uint8_t*>(rows) + n * S->h->qbytes);
+    r->flags = flags;
+    r->nq = n;
+    r->t_submit = now_us();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        r->ticket = S->next_ticket++;
+        S->by_ticket[r->ticket] = r;
+        S->pending.push_back(r);
+        *ticket = r->ticket;
+    }
+    S->cv_work.notify_one();
+    return PK_OK;
+}
+
+int pk_server_wait(pk_server* S, int64_t ticket, int32_t timeout_ms, double* latency_us, double* queue_us) {
+    if (!S) return set_error(PK_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->by_ticket.find(ticket);
+    if (it == S->by_ticket.end()) return set_error(PK_ERR_ARG, "unknown ticket");
+    Request* r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_ms < 0)
+        S->cv_done.wait(lk, pred);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
+        return set_error(PK_ERR_STATE, "request not finished within the timeout");
+    if (latency_us) *latency_us = r->t_done - r->t_submit;
+    if (queue_us) *queue_us = r->t_start - r->t_submit;
+    int status = r->status;
+    S->by_ticket.erase(it);
+    delete r;
+    return status == PK_OK ? PK_OK : set_error(status, "request failed");
+}
+
+int pk_server_stats(pk_server* S, int64_t* out) {
+    if (!S || !out) return set_error(PK_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    out[0] = S->launches;
+    out[1] = S->queries_done;
+    out[2] = S->inserts_done;
+    out[3] = S->max_concurrent;
+    out[4] = (int64_t)S->pending.size();
+    return PK_OK;
+}
+
+int pk_server_destroy(pk_server* S) {
+    if (!S) return PK_OK;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->th.joinable()) S->th.join();
+    for (Slot& s : S->slots) {
+        if (s.ctx.stream) (void)hipStreamSynchronize(s.ctx.stream);
+        free_slot(s);
+    }
+    for (auto& kv : S->by_ticket) delete kv.second;
+    S->h->graphs_alive--;
+    delete S;
+    return PK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+"""The serving loop (pk_server_*) on the GPU.
+
+Requests of mixed size, k and ef are submitted from several threads at once;
+each result must equal the same queries searched alone (the search is exact,
+so the mix cannot change a result).  Streaming inserts go through the loop
+and are visible to the searches admitted after them."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pickle_amd import abi, datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_requests_alone_equal_mixed_and_inserts_visible():
+    n0, extra, dim, M, efc = 20000, 2000, 64, 12, 80
+    base, queries = datasets.gaussian_mixture(n0 + extra, 3000, dim, 32, 41, kind="u8")
+    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n0 + extra)
+    levels = abi.sample_levels(n0 + extra, M, 41)
+    idx.set_base(base[:n0])
+    idx.init_build(levels)
+    idx.insert(np.arange(n0, dtype=np.int32), max_batch=4096)
+    rng = np.random.default_rng(5)
+    reqs, at = [], 0
+    while at < len(queries):
+        m = int(min(len(queries) - at, np.exp(rng.uniform(0, np.log(400)))))
+        reqs.append((at, m, int(rng.choice([1, 10, 32])), int(rng.integers(32, 97))))
+        at += m
+    say("requests", len(reqs))
+    alone = [idx.search(queries[s:s + m], k, ef) for s, m, k, ef in reqs]
+
+    srv = abi.Server(idx, slots=4, slot_queries=512, max_k=32, max_ef=128)
+    results = [None] * len(reqs)
+
+    def client(part):
+        tickets = [(i, srv.submit(queries[reqs[i][0]:reqs[i][0] + reqs[i][1]], reqs[i][2], reqs[i][3]))
+                   for i in part]
+        for i, t in tickets:
+            results[i] = srv.wait(t)
+
+    threads = [threading.Thread(target=client, args=(list(range(j, len(reqs), 4)),)) for j in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    st = srv.stats()
+    say("stats", st)
+    for i, (s, m, k, ef) in enumerate(reqs):
+        ids, ds, cnt, lat, que = results[i]
+        np.testing.assert_array_equal(ids, alone[i][0][:, :k], err_msg=f"request {i}")
+        np.testing.assert_array_equal(ds, alone[i][1][:, :k], err_msg=f"request {i}")
+        np.testing.assert_array_equal(cnt, alone[i][2], err_msg=f"request {i}")
+        assert lat >= que >= 0
+    assert st["queries"] == len(queries) and st["launches"] < len(reqs)
+    assert st["max_concurrent"] >= 2
+
+    # streaming insert, then the new rows find themselves
+    new_ids = np.arange(n0, n0 + extra, dtype=np.int32)
+    t_ins = srv.insert(new_ids, base[n0:])
+    t_q = srv.submit(base[n0:n0 + 500], 1, 64)
+    assert srv.wait(t_ins)[0] is None
+    ids, ds, _, _, _ = srv.wait(t_q)
+    found = float(np.mean(ids[:, 0] == new_ids[:500]))
+    say("inserted rows found as their own nearest neighbour:", found)
+    assert found >= 0.95
+    srv.close()
+    # the loop's post-insert answers equal the eager search of the same graph
+    np.testing.assert_array_equal(ids, idx.search(base[n0:n0 + 500], 1, 64)[0])
+    idx.close()

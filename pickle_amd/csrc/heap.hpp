@@ -73,7 +73,8 @@ struct WaveHeap {
     }
     __device__ __forceinline__ bool full() const { return len >= lcap + (SPILL ? gcap : 0); }
 
-    __device__ void push_par(uint64_t e, int lane, bool leader) {
+    // returns true when the entry became the root
+    __device__ bool push_par(uint64_t e, int lane, bool leader) {
         const int i = uni(len);
         len = i + 1;
         const float d = entry_dist(e);
@@ -92,6 +93,7 @@ struct WaveHeap {
         // before any lane reads them back
         if (SPILL && i >= lcap) __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
+        return stop == t + 1;
     }
 
     // Sift-down reading a three-level subtree per round.  Relative to the
@@ -146,17 +148,25 @@ struct WaveHeap {
         __builtin_amdgcn_wave_barrier();
     }
 
-    // parallel sift-down; needs len <= 128 * kPopSlots
-    __device__ void pop_par(int lane, bool leader) {
+    // parallel sift-down for heaps of up to 128 * SLOTS entries; returns the
+    // new root (undefined when the heap is left empty)
+    __device__ uint64_t pop_par(int lane, bool leader) {
+        if (lcap <= 128) return pop_slots<1>(lane, leader);
+        if (lcap <= 256) return pop_slots<2>(lane, leader);
+        return pop_slots<kPopSlots>(lane, leader);
+    }
+
+    template <int SLOTS>
+    __device__ uint64_t pop_slots(int lane, bool leader) {
         const int n = uni(len) - 1;
         const uint64_t x = get(n);
         len = n;
-        if (n == 0) return;
+        if (n == 0) return x;
         const float xd = entry_dist(x);
-        int pc[kPopSlots];
-        uint32_t pvl[kPopSlots], pvh[kPopSlots];
+        int pc[SLOTS];
+        uint32_t pvl[SLOTS], pvh[SLOTS];
 #pragma unroll
-        for (int s = 0; s < kPopSlots; s++) {
+        for (int s = 0; s < SLOTS; s++) {
             const int node = lane + kWave * s;
             const int l = 2 * node + 1, r = l + 1;
             int c = -1;
@@ -179,12 +189,13 @@ struct WaveHeap {
         }
         __builtin_amdgcn_wave_barrier();
         int cur = 0;
+        uint64_t root = x;
         for (;;) {
             const int s = cur >> 6, ln = cur & 63;
             int c = -1;
             uint32_t lo = 0, hi = 0;
 #pragma unroll
-            for (int k = 0; k < kPopSlots; k++)
+            for (int k = 0; k < SLOTS; k++)
                 if (k == s) {
                     c = readlane(pc[k], ln);
                     lo = (uint32_t)readlane((int)pvl[k], ln);
@@ -193,11 +204,13 @@ struct WaveHeap {
             if (c < 0) break;
             const uint64_t cv = (uint64_t)lo | ((uint64_t)hi << 32);
             if (!below(xd, entry_dist(cv))) break;
+            if (cur == 0) root = cv;
             put(cur, cv, leader);
             cur = c;
         }
         put(cur, x, leader);
         __builtin_amdgcn_wave_barrier();
+        return root;
     }
 
     __device__ __forceinline__ uint64_t top() const { return get(0); }

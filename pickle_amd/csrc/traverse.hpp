@@ -207,14 +207,19 @@ __device__ void beam_level(const DF& df, const LayerDev& L, int vstride, int ent
                 const float dv = readlanef(d, ln);
                 if (W.len < ef || __uint_as_float(wtop_bits) > dv) {  // hnsw.hpp:425-431
                     const uint64_t e = pack_entry(dv, readlane(id, ln));
-                    W.push_par(e, lane, leader);
+                    // W's new root is known without reading it back: the
+                    // entry if it rose to the root, else the old root; after
+                    // an eviction, what the sift-down put at the root
+                    const bool rose = W.push_par(e, lane, leader);
                     if (C.full()) {
                         fail = kFailHeapFull;
                         break;
                     }
                     C.push_par(e, lane, leader);
-                    if (W.len > ef) W.pop_par(lane, leader);
-                    wtop_bits = (uint32_t)W.top();
+                    if (W.len > ef)
+                        wtop_bits = (uint32_t)W.pop_par(lane, leader);
+                    else if (rose)
+                        wtop_bits = (uint32_t)e;
                 }
             }
         }

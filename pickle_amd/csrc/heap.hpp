@@ -73,27 +73,50 @@ struct WaveHeap {
     }
     __device__ __forceinline__ bool full() const { return len >= lcap + (SPILL ? gcap : 0); }
 
-    // returns true when the entry became the root
-    __device__ bool push_par(uint64_t e, int lane, bool leader) {
-        const int i = uni(len);
+    // Push in two phases so two heaps can share one LDS round trip:
+    // push_read fetches the new slot's ancestors (lane j: ancestor j + 1),
+    // push_write decides the stop and moves entries.
+    struct PushState {
+        int i, t;
+        uint64_t v;
+    };
+    __device__ __forceinline__ PushState push_read(int lane) const {
+        PushState p;
+        p.i = uni(len);
+        p.t = 31 - __builtin_clz((unsigned)(p.i + 1));  // number of ancestors
+        const int j = lane + 1;
+        const bool act = j <= p.t;
+        p.v = act ? raw_get(((p.i + 1) >> j) - 1) : 0;
+        return p;
+    }
+    // returns true when the entry became the root; *last (if given) receives
+    // the entry now in the last slot (e, or the parent it displaced)
+    __device__ __forceinline__ bool push_write(const PushState& p, uint64_t e, int lane, bool leader,
+                                               uint64_t* last = nullptr) {
+        const int i = p.i, t = p.t;
         len = i + 1;
         const float d = entry_dist(e);
-        const int t = 31 - __builtin_clz((unsigned)(i + 1));  // number of ancestors
-        const int j = lane + 1;                              // this lane's ancestor
+        const int j = lane + 1;
         const bool act = j <= t;
-        const int aj = act ? ((i + 1) >> j) - 1 : 0;
-        const uint64_t v = act ? raw_get(aj) : 0;
-        const bool up = act && below(entry_dist(v), d);
+        const bool up = act && below(entry_dist(p.v), d);
         const uint64_t stopm = ballot(act && !up);
         const int stop = stopm ? (int)__builtin_ctzll(stopm) + 1 : t + 1;
         __builtin_amdgcn_wave_barrier();
-        if (act && j < stop) raw_put(((i + 1) >> (j - 1)) - 1, v);
+        if (act && j < stop) raw_put(((i + 1) >> (j - 1)) - 1, p.v);
         if (leader) raw_put(((i + 1) >> (stop - 1)) - 1, e);
+        if (last)
+            *last = stop == 1 ? e
+                              : ((uint64_t)(uint32_t)readlane((int)(uint32_t)p.v, 0) |
+                                 ((uint64_t)(uint32_t)readlane((int)(uint32_t)(p.v >> 32), 0) << 32));
         // device-memory slots written by several lanes: complete the stores
         // before any lane reads them back
         if (SPILL && i >= lcap) __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
         return stop == t + 1;
+    }
+    __device__ bool push_par(uint64_t e, int lane, bool leader) {
+        const PushState p = push_read(lane);
+        return push_write(p, e, lane, leader);
     }
 
     // Sift-down reading a three-level subtree per round.  Relative to the
@@ -150,16 +173,18 @@ struct WaveHeap {
 
     // parallel sift-down for heaps of up to 128 * SLOTS entries; returns the
     // new root (undefined when the heap is left empty)
-    __device__ uint64_t pop_par(int lane, bool leader) {
-        if (lcap <= 128) return pop_slots<1>(lane, leader);
-        if (lcap <= 256) return pop_slots<2>(lane, leader);
-        return pop_slots<kPopSlots>(lane, leader);
+    __device__ uint64_t pop_par(int lane, bool leader) { return pop_with(get(uni(len) - 1), lane, leader); }
+
+    // pop when the caller already holds the entry in the last slot (x)
+    __device__ uint64_t pop_with(uint64_t x, int lane, bool leader) {
+        if (lcap <= 128) return pop_slots<1>(x, lane, leader);
+        if (lcap <= 256) return pop_slots<2>(x, lane, leader);
+        return pop_slots<kPopSlots>(x, lane, leader);
     }
 
     template <int SLOTS>
-    __device__ uint64_t pop_slots(int lane, bool leader) {
+    __device__ uint64_t pop_slots(const uint64_t x, int lane, bool leader) {
         const int n = uni(len) - 1;
-        const uint64_t x = get(n);
         len = n;
         if (n == 0) return x;
         const float xd = entry_dist(x);

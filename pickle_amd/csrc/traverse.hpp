@@ -207,17 +207,23 @@ __device__ void beam_level(const DF& df, const LayerDev& L, int vstride, int ent
                 const float dv = readlanef(d, ln);
                 if (W.len < ef || __uint_as_float(wtop_bits) > dv) {  // hnsw.hpp:425-431
                     const uint64_t e = pack_entry(dv, readlane(id, ln));
-                    // W's new root is known without reading it back: the
-                    // entry if it rose to the root, else the old root; after
-                    // an eviction, what the sift-down put at the root
-                    const bool rose = W.push_par(e, lane, leader);
                     if (C.full()) {
                         fail = kFailHeapFull;
                         break;
                     }
-                    C.push_par(e, lane, leader);
+                    // W and C are pushed with one LDS round trip for both
+                    // ancestor reads.  W's new root is known without reading
+                    // it back: the entry if it rose to the root, else the old
+                    // root; after the eviction, what the sift-down put at
+                    // the root -- and the evicted-into-root entry (the last
+                    // slot's) is known from the push.
+                    const auto pw = W.push_read(lane);
+                    const auto pc = C.push_read(lane);
+                    uint64_t last;
+                    const bool rose = W.push_write(pw, e, lane, leader, &last);
+                    C.push_write(pc, e, lane, leader);
                     if (W.len > ef)
-                        wtop_bits = (uint32_t)W.pop_par(lane, leader);
+                        wtop_bits = (uint32_t)W.pop_with(last, lane, leader);
                     else if (rose)
                         wtop_bits = (uint32_t)e;
                 }

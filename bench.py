@@ -45,6 +45,8 @@ WORKLOADS = {
                          "M=32, efC=256, 10k queries, k=10, CPU-built graph", n=1_000_000, nq=10_000),
     "gm100k_f32": dict(desc="100k x 128 float32 (rounded), 64-cluster gaussian mixture (seed 0), L2, M=16, "
                             "efC=200, 1k queries, k=10", n=100_000, nq=1_000),
+    "vmf1m_f32": dict(desc="1M x 96 float32 unit vectors, 256-cluster vMF-like mixture (seed 2), IP, M=24, "
+                           "efC=200, 50k queries", n=1_000_000, nq=50_000, build="gpu"),
     "gm100m_u8": dict(desc="100M x 128 uint8, 1024-cluster gaussian mixture quantised to [0,255] (seed 4), L2, "
                            "M=32, efC=256", n=100_000_000, nq=100_000, build="gpu"),
     "gm10m_f32": dict(desc="10M x 128 float32, 64-cluster gaussian mixture (seed 3), L2, M=32, efC=200, 100k "
@@ -166,13 +168,30 @@ def _short_kernel(name):
     return name[:60]
 
 
-def _counter_per_dispatch(directory, counter):
-    out = {}
+def _counter_per_dispatch(directory, counter, last=None):
+    """Counter values per kernel, in dispatch order; with `last`, only the
+    last `last` dispatches of k_search (the timed region of a bench run)."""
+    rows = []
     for r in _csv_rows(directory, "counter_collection.csv"):
-        if r["Counter_Name"] != counter:
-            continue
-        out.setdefault(_short_kernel(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        if r["Counter_Name"] == counter:
+            rows.append((int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]), float(r["Counter_Value"])))
+    rows.sort()
+    out = {}
+    for _, key, v in rows:
+        out.setdefault(key, []).append(v)
+    if last:
+        for key in out:
+            if key.startswith("k_search"):
+                out[key] = out[key][-last:]
     return out
+
+
+def _timed_search_ms(directory, last):
+    """Average duration of the last `last` k_search dispatches in a kernel trace."""
+    rows = [r for r in _csv_rows(directory, "kernel_trace.csv") if "k_search" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    sel = rows[-last:]
+    return sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sel) / len(sel) / 1e6
 
 
 def summarize(args):
@@ -197,16 +216,19 @@ def summarize(args):
                 search_avg = avg
             body.append(f"| `{name}` | {r['Calls']} | {avg:.4f} ms | {float(r['MinNs']) / 1e6:.4f} ms | "
                         f"{float(r['MaxNs']) / 1e6:.4f} ms |")
-        body += ["", f"bench.py in the same run: {roof['kernel_ms']} ms per k_search launch (HIP events on the "
-                 f"index stream; trace average {search_avg:.4f} ms), {bl['value']:,.0f} QPS, algorithmic "
+        timed_ms = _timed_search_ms(args.summarize_trace, bl["steps"])
+        body += ["", f"The last {bl['steps']} k_search dispatches are the timed region: {timed_ms:.4f} ms each on "
+                 f"average in the trace (the others are the ef sweep, the counter pass and the autotuner's trials).",
+                 f"bench.py in the same run: {roof['kernel_ms']} ms per k_search launch (HIP events on the "
+                 f"index stream), {bl['value']:,.0f} QPS, algorithmic "
                  f"{roof['algorithmic_bytes_per_launch'] / 1e9:.3f} GB per launch -> {roof['achieved']} GB/s "
                  f"= {roof['frac'] * 100:.2f} % of 8 TB/s.",
                  f"Per query: {roof['per_query']['hops']} hops, {roof['per_query']['neighbors']} adjacency "
                  f"entries scanned, {roof['per_query']['dists']} distances."]
     if args.summarize_pmc:
         fdir, wdir = args.summarize_pmc
-        fetch = _counter_per_dispatch(fdir, "FETCH_SIZE")
-        write = _counter_per_dispatch(wdir, "WRITE_SIZE")
+        fetch = _counter_per_dispatch(fdir, "FETCH_SIZE", last=bl["steps"])
+        write = _counter_per_dispatch(wdir, "WRITE_SIZE", last=bl["steps"])
         kib = CALIB_BYTES / 1024.0
         ff = (sum(fetch["k_copy16"]) / len(fetch["k_copy16"])) / kib
         fw = (sum(write["k_copy16"]) / len(write["k_copy16"])) / kib
@@ -216,7 +238,7 @@ def summarize(args):
         fb, wb = f_kib * 1024 / ff, w_kib * 1024 / fw
         alg = roof["algorithmic_bytes_per_launch"]
         body += ["", "Hardware byte counters (two `--pmc` passes, KiB per dispatch, averaged over the "
-                 "dispatches of each kernel):", "",
+                 "dispatches of each kernel; for k_search the timed region's last dispatches):", "",
                  "| Kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |", "|---|---|---|---|"]
         for key in sorted(set(fetch) | set(write)):
             fv, wv = fetch.get(key, [0]), write.get(key, [0])
@@ -347,6 +369,120 @@ def run_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def host_ground_truth(base, queries, k, metric):
+    """Exact top-k on the host: FP32 GEMM blocks, top 2k candidates, FP64
+    recheck (for k larger than the GPU brute force keeps)."""
+    B = np.asarray(base, np.float32)
+    bn = (B.astype(np.float64) ** 2).sum(1) if metric == "l2" else None
+    out = np.empty((len(queries), k), np.int64)
+    for s0 in range(0, len(queries), 256):
+        Q = np.asarray(queries[s0:s0 + 256], np.float32)
+        dots = Q @ B.T
+        d = (bn[None, :] - 2 * dots) if metric == "l2" else -dots
+        cand = np.argpartition(d, 2 * k, axis=1)[:, :2 * k]
+        for i in range(len(Q)):
+            c = cand[i]
+            A, q = B[c].astype(np.float64), Q[i].astype(np.float64)
+            d64 = ((A - q) ** 2).sum(1) if metric == "l2" else -(A @ q)
+            out[s0 + i] = c[np.lexsort((c, d64))][:k]
+    return out
+
+
+def run_uneven(args):
+    """Uneven-batch persistent search on a GPU-built graph: 1M x 96 unit
+    vectors (vMF-like mixture, IP), the GPU batched build compared with the
+    reference's CPU build by recall vs ef, then one launch of all queries with
+    per-query ef log-uniform in [16, 512] and k in {1, 10, 100}, autotuned
+    and captured in a HIP graph; the timed steps replay the graph."""
+    from pickle_amd import abi, datasets
+
+    wl = WORKLOADS[args.workload]
+    n, nq = args.n or wl["n"], args.nq or wl["nq"]
+    base, queries, params = datasets.workload(args.workload, n=n, nq=nq)
+    dim, metric, M, efc = base.shape[1], params["metric"], params["M"], params["efc"]
+    mcode = abi.L2 if metric == "l2" else abi.IP
+    threads = host_threads()
+    binary, isa = ref_binary()
+    work = args.graph_cache or tempfile.mkdtemp(prefix="pk_bench_")
+    os.makedirs(work, exist_ok=True)
+    base_path, graph_path = os.path.join(work, "uneven.base"), os.path.join(work, "uneven_cpu.graph")
+    base.tofile(base_path)
+    # graphs: GPU batched build and the reference's CPU build
+    idx = abi.Index(base.dtype, mcode, dim, M, efc, n)
+    tb = time.perf_counter()
+    idx.build(base, max_batch=args.insert_batch, seed=args.seed,
+              progress=lambda d, t: log(f"  GPU build {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
+    idx.sync()
+    gpu_build_s = time.perf_counter() - tb
+    log(f"GPU batched build in {gpu_build_s:.1f}s")
+    cpu_info = cpu_build(binary, base_path, n, dim, "f32", metric, M, efc, threads, graph_path)
+    log(f"reference CPU build ({isa}, {threads} threads) in {cpu_info['build_seconds']:.1f}s")
+    cidx = abi.Index(base.dtype, mcode, dim, M, efc, n)
+    cidx.set_base(base)
+    cidx.load_file(graph_path)
+
+    # exact ground truth (top 100) on a query subset
+    ns = min(nq, args.gt_queries or 1000)
+    t1 = time.time()
+    gt = host_ground_truth(base, queries[:ns], 100, metric)
+    log(f"host ground truth (top 100, {ns} queries) in {time.time() - t1:.1f}s")
+    curve = []
+    for ef in (16, 32, 64, 128, 256, 512):
+        rg = recall_at_k(idx.search(queries[:ns], 10, ef)[0], gt, 10)
+        rc = recall_at_k(cidx.search(queries[:ns], 10, ef)[0], gt, 10)
+        curve.append([ef, round(rg, 4), round(rc, 4)])
+        log(f"recall@10 ef={ef}: GPU-built {rg:.4f}, reference-built {rc:.4f}")
+    cidx.close()
+
+    # the uneven launch
+    rng = np.random.default_rng(args.seed)
+    efs = np.exp(rng.uniform(np.log(16), np.log(512), nq)).astype(np.int32)
+    ks = rng.choice([1, 10, 100], nq).astype(np.int32)
+    efs = np.maximum(efs, ks)
+    d_q = abi.DeviceBuffer.from_array(queries)
+    d_k, d_e = abi.DeviceBuffer.from_array(ks), abi.DeviceBuffer.from_array(efs)
+    d_ids, d_d = abi.DeviceBuffer(nq * 100 * 4), abi.DeviceBuffer(nq * 100 * 4)
+    d_c = abi.DeviceBuffer(nq * 4)
+    tuned = None
+    if not args.no_autotune:
+        tuned = idx.autotune(d_q, min(nq, 4096), 10, int(np.median(efs)))
+        log(f"autotuned at the median ef {int(np.median(efs))}: {tuned}")
+    graph = idx.search_graph(d_q, nq, d_k, d_e, 100, d_ids, d_d, d_c, ef_max=int(efs.max()))
+    for _ in range(args.warmup):
+        graph.launch()
+    idx.sync()
+    tw0 = time.perf_counter()
+    idx.timer_begin()
+    for _ in range(args.steps):
+        graph.launch()
+    ev_ms = idx.timer_end()
+    wall = time.perf_counter() - tw0
+    graph.close()
+    qps = nq * args.steps / wall
+    ids = d_ids.to_array(np.int32, (nq, 100))
+    rec = {}
+    for kk in (1, 10, 100):
+        sel = np.where(ks[:ns] == kk)[0]
+        hit = sum(len(set(ids[i, :kk].tolist()) & set(gt[i, :kk].tolist())) for i in sel)
+        rec[f"k={kk}"] = round(hit / (kk * max(1, len(sel))), 4)
+    log(f"uneven launch: {nq} queries, {args.steps} graph replays: {qps:,.0f} QPS, kernel "
+        f"{ev_ms / args.steps:.3f} ms; recall by k {rec}")
+    line = {
+        "metric": "QPS of one uneven launch (per-query ef log-uniform [16, 512], k in {1, 10, 100}), HIP graph",
+        "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": wl["desc"], "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc,
+                   "ef": "log-uniform [16, 512] per query", "k": "{1, 10, 100} per query",
+                   "recall_by_k": rec, "recall_queries": ns, "gpu_build_seconds": round(gpu_build_s, 2),
+                   "reference_cpu_build_seconds": round(cpu_info["build_seconds"], 2), "reference_build_threads":
+                   threads, "recall10_vs_ef_gpu_built_vs_reference_built": curve, "autotuned": tuned,
+                   "kernel_ms": round(ev_ms / args.steps, 4)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def recall_at_k(ids, gt, k):
     """benchmark/hnsw_pickle.cpp:94-112: a returned id counts if it is one of
     the first k ground-truth labels (ties are not credited)."""
@@ -379,6 +515,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", choices=["none", "f16", "i8"], default=None,
                     help="search a low-precision traversal copy + FP32 re-rank (float workloads)")
+    ap.add_argument("--no-autotune", action="store_true", help="time with the default tuning")
     ap.add_argument("--base-file", default=None, help=".npy / .fbin / .u8bin / .i8bin base vectors instead of "
                                                       "a synthetic workload (with --query-file)")
     ap.add_argument("--query-file", default=None)
@@ -386,6 +523,8 @@ def main():
     ap.add_argument("--M", type=int, default=32, help="with --base-file")
     ap.add_argument("--efc", type=int, default=200, help="with --base-file")
     ap.add_argument("--serve", action="store_true", help="open-loop serving run through the request loop")
+    ap.add_argument("--uneven", action="store_true",
+                    help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--insert-share", type=float, default=0.05)
     ap.add_argument("--load", type=float, default=0.6, help="offered load as a fraction of measured capacity")
@@ -401,6 +540,10 @@ def main():
 
     if args.summarize_trace or args.summarize_pmc:
         summarize(args)
+        return
+
+    if args.uneven:
+        run_uneven(args)
         return
 
     if args.serve:
@@ -548,6 +691,33 @@ def main():
             log(f"[{which}] no ef reached recall {target}; using ef={chosen}")
         return chosen, [x[1] for x in out if x[0] == chosen][0], out
 
+    tuned = {}
+
+    def autotune(which, ef):
+        """The library's autotuner on the first 4,096 queries at this ef (its
+        choice does not change results); skipped with --no-autotune."""
+        if args.no_autotune:
+            return
+        h, _ = backend(which)
+        ns = min(nq, 4096)
+        if which == "fp32":
+            dq, hq = d_q, None
+        else:
+            qc = queries[:ns]
+            if prec == "f16":
+                qc = qc.astype(np.float16)
+            else:
+                from pickle_amd import precision as P
+                off, sc = P.int8_quantizer(base, mcode)
+                inv = (np.float32(1.0) / sc).astype(np.float32)
+                qc = np.clip(np.rint(((qc - off).astype(np.float32) * inv).astype(np.float32)), -127, 127)
+                qc = qc.astype(np.int8)
+            hq = abi.DeviceBuffer.from_array(np.ascontiguousarray(qc))
+            dq = hq
+        t = h.autotune(dq, ns, k, ef)
+        tuned[which] = t
+        log(f"[{which}] autotuned at ef={ef}: {t}")
+
     def timed(which, ef):
         h, run = backend(which)
         for _ in range(args.warmup):
@@ -568,6 +738,7 @@ def main():
         chosen, recall, sweep_out = sweep("fp32", args.recall_target, efs)
     else:
         ef32, r32, sw32 = sweep("fp32", args.recall_target, efs)
+        autotune("fp32", ef32)
         wall32, ks32 = timed("fp32", ef32)
         qps32 = nq * args.steps / wall32
         # the precision form at equal recall: the first ef within 0.002 of the FP32 recall
@@ -596,6 +767,7 @@ def main():
     khz = abi.wall_clock_khz()
     lat_ms = ctr["latency_ticks"].astype(np.float64) / khz
 
+    autotune("fp32" if main_be == "fp32" else "rerank", chosen)
     wall, kernel_s = timed(main_be, chosen)
     qps = nq * args.steps / wall
     achieved = alg_bytes / kernel_s / 1e9
@@ -664,6 +836,8 @@ def main():
            "batch_latency_ms": round(wall / args.steps * 1e3, 4), "sweep_ef_recall_qps": sweep_out}
     if comparison:
         cfg["precision_comparison"] = comparison
+    if tuned:
+        cfg["autotuned"] = tuned
     line = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,

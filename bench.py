@@ -525,9 +525,9 @@ def main():
     ap.add_argument("--serve", action="store_true", help="open-loop serving run through the request loop")
     ap.add_argument("--uneven", action="store_true",
                     help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
-    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--insert-share", type=float, default=0.05)
-    ap.add_argument("--load", type=float, default=0.6, help="offered load as a fraction of measured capacity")
+    ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",
                     help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")

@@ -179,8 +179,10 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
  * arrival order (requests are not split), so requests that arrive while
  * others run start as slots free up, and slots overlap on the device.  Each
  * request has its own k <= max_k and k <= ef <= max_ef.  Inserts queue in
- * the same order, run alone when no launch is in flight, and are visible to
- * every search admitted after them (the index must come from
+ * the same order and are applied 256 rows at a time while no launch is in
+ * flight, alternating with rounds of searches; searches running meanwhile
+ * see the rows inserted so far, and every search submitted after the
+ * insert's pk_server_wait returns sees all of them (the index must come from
  * pk_index_init_build).  All calls are thread-safe.  While a server exists
  * the index's layers cannot be replaced. */
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);

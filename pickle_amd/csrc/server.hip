@@ -11,11 +11,14 @@
 // frees, and slots overlap on the device.  One launch carries per-query k
 // and ef arrays, so requests with different k / ef share it.
 //
-// Inserts go through the same loop: an insert waits until no launch is in
-// flight, runs alone (the reference's Insert is not safe beside AnnSearch,
-// include/hnsw.hpp:247-302), and every search admitted after it sees the new
-// nodes.  A query's result depends only on the graph it ran on, so it is the
-// same alone and in any mix.
+// Inserts go through the same loop.  The reference's Insert is not safe
+// beside AnnSearch (include/hnsw.hpp:247-302), so an insert is applied in
+// steps of kInsertChunk rows, each while no launch is in flight, alternating
+// with rounds of search launches: a large insert does not stall searches for
+// its whole length.  Searches that run between steps see the rows inserted
+// so far; every search submitted after the insert's wait returns sees all of
+// them.  A query's result depends only on the graph it ran on, so with no
+// insert in progress it is the same alone and in any mix.
 //
 // All HIP work happens on the dispatcher thread.
 #include <algorithm>
@@ -44,6 +47,8 @@ namespace {
 double now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+constexpr int64_t kInsertChunk = 256;  // rows applied per exclusive insert step
 
 struct Request {
     int64_t ticket = 0;
@@ -90,6 +95,9 @@ struct pk_server {
     int64_t next_ticket = 1;
     bool stop = false;
     int device = 0;
+    Request* active = nullptr;  // insert being applied
+    int64_t active_pos = 0;
+    bool insert_turn = true;
     std::thread th;
     // statistics
     int64_t launches = 0, queries_done = 0, inserts_done = 0, max_concurrent = 0;
@@ -196,34 +204,48 @@ void dispatcher(pk_server* S) {
         int busy = 0;
         for (Slot& s : S->slots) busy += s.busy;
         if (progress) S->cv_done.notify_all();
-        if (S->stop && S->pending.empty() && busy == 0) break;
-        // an insert at the head of the queue runs alone
-        if (!S->pending.empty() && S->pending.front()->is_insert) {
-            if (busy == 0) {
-                Request* r = S->pending.front();
-                S->pending.pop_front();
-                r->t_start = now_us();
-                lk.unlock();
-                // rows by external id (contiguous runs in one copy each), then Insert
-                int rc = PK_OK;
-                const int64_t n = (int64_t)r->ext_ids.size();
-                for (int64_t i = 0; i < n && rc == PK_OK;) {
-                    int64_t j = i + 1;
-                    while (j < n && r->ext_ids[j] == r->ext_ids[j - 1] + 1) j++;
-                    rc = pk_index_set_base(S->h, r->queries.data() + i * S->h->qbytes, r->ext_ids[i], j - i);
-                    i = j;
-                }
-                if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data(), n, 4096, r->flags);
-                lk.lock();
+        if (S->stop && S->pending.empty() && busy == 0 && !S->active) break;
+        // inserts: the head-of-queue insert becomes the active one and is
+        // applied kInsertChunk rows at a time while no launch is in flight,
+        // alternating with one round of search admissions
+        if (!S->active && !S->pending.empty() && S->pending.front()->is_insert) {
+            S->active = S->pending.front();
+            S->pending.pop_front();
+            S->active->t_start = now_us();
+            S->active_pos = 0;
+            S->insert_turn = true;
+        }
+        if (S->active && busy == 0 && S->insert_turn) {
+            Request* r = S->active;
+            const int64_t n = (int64_t)r->ext_ids.size();
+            const int64_t e = std::min<int64_t>(n, S->active_pos + kInsertChunk);
+            const int64_t b0 = S->active_pos;
+            lk.unlock();
+            // rows by external id (contiguous runs in one copy each), then Insert
+            int rc = PK_OK;
+            for (int64_t i = b0; i < e && rc == PK_OK;) {
+                int64_t j = i + 1;
+                while (j < e && r->ext_ids[j] == r->ext_ids[j - 1] + 1) j++;
+                rc = pk_index_set_base(S->h, r->queries.data() + i * S->h->qbytes, r->ext_ids[i], j - i);
+                i = j;
+            }
+            if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data() + b0, e - b0, 4096, r->flags);
+            lk.lock();
+            S->active_pos = e;
+            S->insert_turn = S->pending.empty() || S->pending.front()->is_insert;
+            if (rc != PK_OK || e == n) {
                 r->status = rc;
                 r->t_done = now_us();
                 r->done = true;
                 S->inserts_done++;
+                S->active = nullptr;
                 S->cv_done.notify_all();
-                continue;
             }
-        } else {
-            // admission: free slots take waiting requests in arrival order
+            continue;
+        }
+        {
+            // admission: free slots take waiting searches in arrival order
+            // (an insert waiting behind the active one blocks those after it)
             for (Slot& s : S->slots) {
                 if (s.busy || S->pending.empty() || S->pending.front()->is_insert) continue;
                 int64_t nq = 0;
@@ -242,6 +264,7 @@ void dispatcher(pk_server* S) {
                 if (rc) complete_slot(S, s, rc);
                 progress = true;
             }
+            if (S->active) S->insert_turn = true;  // the next chunk once these launches finish
             busy = 0;
             for (Slot& s : S->slots) busy += s.busy;
             S->max_concurrent = std::max<int64_t>(S->max_concurrent, busy);

@@ -66,8 +66,10 @@ def test_requests_alone_equal_mixed_and_inserts_visible():
     # streaming insert, then the new rows find themselves
     new_ids = np.arange(n0, n0 + extra, dtype=np.int32)
     t_ins = srv.insert(new_ids, base[n0:])
-    t_q = srv.submit(base[n0:n0 + 500], 1, 64)
+    t_mid = srv.submit(queries[:300], 10, 64)  # may run between the insert's steps
     assert srv.wait(t_ins)[0] is None
+    assert srv.wait(t_mid)[2].min() == 10
+    t_q = srv.submit(base[n0:n0 + 500], 1, 64)  # submitted after the insert finished
     ids, ds, _, _, _ = srv.wait(t_q)
     found = float(np.mean(ids[:, 0] == new_ids[:500]))
     say("inserted rows found as their own nearest neighbour:", found)

@@ -328,6 +328,7 @@ def run_serving(args):
                 q_at += m
         tickets.append(None)
 
+    slog = []
     th = threading.Thread(target=producer)
     th.start()
     j = 0
@@ -341,10 +342,17 @@ def run_serving(args):
         kind, m, tk = t
         _, _, _, lat, que = srv.wait(tk)
         done.append((kind, m, lat, que))
+        if args.serve_log:
+            slog.append({"i": j - 1, "kind": kind, "size": m, "latency_us": round(lat, 1), "queue_us": round(que, 1),
+                         "done_s": round(time.perf_counter() - t0, 6)})
     th.join()
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
+    if args.serve_log:
+        with open(args.serve_log, "w") as f:
+            for rec in slog:
+                f.write(json.dumps(rec) + "\n")
     lat_s = np.array([d[2] for d in done if d[0] == "search"]) / 1e3
     lat_i = np.array([d[2] for d in done if d[0] == "insert"]) / 1e3
     nq_done = sum(d[1] for d in done if d[0] == "search")
@@ -529,6 +537,7 @@ def main():
     ap.add_argument("--insert-share", type=float, default=0.05)
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--serve-log", default=None, help="per-request latencies (JSON lines) of a serving run")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",
                     help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")
     ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),

@@ -178,12 +178,12 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
  * staging for `slot_queries` queries; a free slot takes waiting requests in
  * arrival order (requests are not split), so requests that arrive while
  * others run start as slots free up, and slots overlap on the device.  Each
- * request has its own k <= max_k and k <= ef <= max_ef.  Inserts queue in
- * the same order and are applied 256 rows at a time while no launch is in
- * flight, alternating with rounds of searches; searches running meanwhile
- * see the rows inserted so far, and every search submitted after the
- * insert's pk_server_wait returns sees all of them (the index must come from
- * pk_index_init_build).  All calls are thread-safe.  While a server exists
+ * request has its own k <= max_k and k <= ef <= max_ef.  Inserts keep their
+ * own arrival order and run on a second thread beside the searches (the
+ * index must come from pk_index_init_build): as with the reference's
+ * concurrent Insert and AnnSearch, a search running meanwhile may see part
+ * of an insert, and every search submitted after the insert's
+ * pk_server_wait returns sees all of it.  All calls are thread-safe.  While a server exists
  * the index's layers cannot be replaced. */
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
 /* Queues nq queries (host rows, copied at once); results go to the caller's

@@ -11,16 +11,16 @@
 // frees, and slots overlap on the device.  One launch carries per-query k
 // and ef arrays, so requests with different k / ef share it.
 //
-// Inserts go through the same loop.  The reference's Insert is not safe
-// beside AnnSearch (include/hnsw.hpp:247-302), so an insert is applied in
-// steps of kInsertChunk rows, each while no launch is in flight, alternating
-// with rounds of search launches: a large insert does not stall searches for
-// its whole length.  Searches that run between steps see the rows inserted
-// so far; every search submitted after the insert's wait returns sees all of
-// them.  A query's result depends only on the graph it ran on, so with no
+// Inserts queue in their own arrival order and run on a second thread, on
+// the index's own stream, beside the search launches (see insert_worker for
+// why that is memory-safe).  As with the reference's concurrent Insert and
+// AnnSearch, a search running while an insert is applied may see some of its
+// edges; every search submitted after the insert's wait returns sees all of
+// it.  A query's result depends only on the graph it ran on, so with no
 // insert in progress it is the same alone and in any mix.
 //
-// All HIP work happens on the dispatcher thread.
+// Search launches are issued by the dispatcher thread, inserts by the insert
+// thread.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -47,8 +47,6 @@ namespace {
 double now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-constexpr int64_t kInsertChunk = 256;  // rows applied per exclusive insert step
 
 struct Request {
     int64_t ticket = 0;
@@ -90,15 +88,13 @@ struct pk_server {
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Request*> pending;
+    std::deque<Request*> pending;         // searches, in arrival order
+    std::deque<Request*> pending_insert;  // inserts, in arrival order
     std::unordered_map<int64_t, Request*> by_ticket;
     int64_t next_ticket = 1;
     bool stop = false;
     int device = 0;
-    Request* active = nullptr;  // insert being applied
-    int64_t active_pos = 0;
-    bool insert_turn = true;
-    std::thread th;
+    std::thread th, th_insert;
     // statistics
     int64_t launches = 0, queries_done = 0, inserts_done = 0, max_concurrent = 0;
 };
@@ -204,53 +200,14 @@ void dispatcher(pk_server* S) {
         int busy = 0;
         for (Slot& s : S->slots) busy += s.busy;
         if (progress) S->cv_done.notify_all();
-        if (S->stop && S->pending.empty() && busy == 0 && !S->active) break;
-        // inserts: the head-of-queue insert becomes the active one and is
-        // applied kInsertChunk rows at a time while no launch is in flight,
-        // alternating with one round of search admissions
-        if (!S->active && !S->pending.empty() && S->pending.front()->is_insert) {
-            S->active = S->pending.front();
-            S->pending.pop_front();
-            S->active->t_start = now_us();
-            S->active_pos = 0;
-            S->insert_turn = true;
-        }
-        if (S->active && busy == 0 && S->insert_turn) {
-            Request* r = S->active;
-            const int64_t n = (int64_t)r->ext_ids.size();
-            const int64_t e = std::min<int64_t>(n, S->active_pos + kInsertChunk);
-            const int64_t b0 = S->active_pos;
-            lk.unlock();
-            // rows by external id (contiguous runs in one copy each), then Insert
-            int rc = PK_OK;
-            for (int64_t i = b0; i < e && rc == PK_OK;) {
-                int64_t j = i + 1;
-                while (j < e && r->ext_ids[j] == r->ext_ids[j - 1] + 1) j++;
-                rc = pk_index_set_base(S->h, r->queries.data() + i * S->h->qbytes, r->ext_ids[i], j - i);
-                i = j;
-            }
-            if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data() + b0, e - b0, 4096, r->flags);
-            lk.lock();
-            S->active_pos = e;
-            S->insert_turn = S->pending.empty() || S->pending.front()->is_insert;
-            if (rc != PK_OK || e == n) {
-                r->status = rc;
-                r->t_done = now_us();
-                r->done = true;
-                S->inserts_done++;
-                S->active = nullptr;
-                S->cv_done.notify_all();
-            }
-            continue;
-        }
+        if (S->stop && S->pending.empty() && busy == 0) break;
         {
-            // admission: free slots take waiting searches in arrival order
-            // (an insert waiting behind the active one blocks those after it)
+            // admission: free slots take waiting searches in arrival order;
+            // searches are not held behind inserts
             for (Slot& s : S->slots) {
-                if (s.busy || S->pending.empty() || S->pending.front()->is_insert) continue;
+                if (s.busy || S->pending.empty()) continue;
                 int64_t nq = 0;
-                while (!S->pending.empty() && !S->pending.front()->is_insert &&
-                       nq + S->pending.front()->nq <= S->slot_queries) {
+                while (!S->pending.empty() && nq + S->pending.front()->nq <= S->slot_queries) {
                     Request* r = S->pending.front();
                     S->pending.pop_front();
                     r->t_start = now_us();
@@ -264,7 +221,6 @@ void dispatcher(pk_server* S) {
                 if (rc) complete_slot(S, s, rc);
                 progress = true;
             }
-            if (S->active) S->insert_turn = true;  // the next chunk once these launches finish
             busy = 0;
             for (Slot& s : S->slots) busy += s.busy;
             S->max_concurrent = std::max<int64_t>(S->max_concurrent, busy);
@@ -279,6 +235,45 @@ void dispatcher(pk_server* S) {
         } else {
             S->cv_work.wait_for(lk, std::chrono::milliseconds(50));
         }
+    }
+}
+
+// Inserts run on their own thread and the index's own stream, beside the
+// search launches: memory-safe because the layers never move (capacity is
+// fixed by init_build), searches take a layer's node count from the
+// published table -- new ids stay out of reach until their batch is
+// published, after its kernels finish -- and AddEdge only writes valid ids
+// (or -1) into rows, whose degrees searches clamp to the allocation.
+void insert_worker(pk_server* S) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (true) {
+        S->cv_work.wait_for(lk, std::chrono::milliseconds(50),
+                            [&] { return S->stop || !S->pending_insert.empty(); });
+        if (S->pending_insert.empty()) {
+            if (S->stop) break;
+            continue;
+        }
+        Request* r = S->pending_insert.front();
+        S->pending_insert.pop_front();
+        r->t_start = now_us();
+        lk.unlock();
+        // rows by external id (contiguous runs in one copy each), then Insert
+        int rc = PK_OK;
+        const int64_t n = (int64_t)r->ext_ids.size();
+        for (int64_t i = 0; i < n && rc == PK_OK;) {
+            int64_t j = i + 1;
+            while (j < n && r->ext_ids[j] == r->ext_ids[j - 1] + 1) j++;
+            rc = pk_index_set_base(S->h, r->queries.data() + i * S->h->qbytes, r->ext_ids[i], j - i);
+            i = j;
+        }
+        if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data(), n, 4096, r->flags);
+        lk.lock();
+        r->status = rc;
+        r->t_done = now_us();
+        r->done = true;
+        S->inserts_done++;
+        S->cv_done.notify_all();
     }
 }
 
@@ -309,6 +304,7 @@ int pk_server_create(pk_index* h, int slots, int slot_queries, int max_k, int ma
     (void)hipGetDevice(&S->device);
     h->graphs_alive++;  // the slots' launches read the layers: no reallocation meanwhile
     S->th = std::thread(dispatcher, S);
+    S->th_insert = std::thread(insert_worker, S);
     *out = S;
     return PK_OK;
 }
@@ -355,10 +351,10 @@ int pk_server_insert(pk_server* S, const int32_t* ext_ids, const void* rows, int
         std::lock_guard<std::mutex> g(S->mu);
         r->ticket = S->next_ticket++;
         S->by_ticket[r->ticket] = r;
-        S->pending.push_back(r);
+        S->pending_insert.push_back(r);
         *ticket = r->ticket;
     }
-    S->cv_work.notify_one();
+    S->cv_work.notify_all();
     return PK_OK;
 }
 
@@ -388,7 +384,7 @@ int pk_server_stats(pk_server* S, int64_t* out) {
     out[1] = S->queries_done;
     out[2] = S->inserts_done;
     out[3] = S->max_concurrent;
-    out[4] = (int64_t)S->pending.size();
+    out[4] = (int64_t)(S->pending.size() + S->pending_insert.size());
     return PK_OK;
 }
 
@@ -400,6 +396,7 @@ int pk_server_destroy(pk_server* S) {
     }
     S->cv_work.notify_all();
     if (S->th.joinable()) S->th.join();
+    if (S->th_insert.joinable()) S->th_insert.join();
     for (Slot& s : S->slots) {
         if (s.ctx.stream) (void)hipStreamSynchronize(s.ctx.stream);
         free_slot(s);

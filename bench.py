@@ -201,7 +201,7 @@ def summarize(args):
     cfg, roof = bl["config"], bl["roofline"]
     head = [f"# k_search — {cfg['workload']}", "",
             f"ef={cfg['ef']}, k={cfg['k']}, {cfg['nq']} queries per launch, recall@10 = {cfg['recall@10']}", "",
-            "Command (GPU box, committed files only):", ""]
+            "Commands (committed files only):", ""]
     head += ["    " + c for c in args.cmd]
     body = []
     if args.summarize_trace:
@@ -563,7 +563,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (run with --gpus 1). "
-              "Multi-GPU sharding is described in DESIGN.md.", file=sys.stderr)
+              "Sharding across GPUs is a design, not part of this measurement.", file=sys.stderr)
         sys.exit(2)
 
     from pickle_amd import abi, datasets

@@ -642,12 +642,15 @@ def main():
     gt_nq = min(nq, args.gt_queries) if args.gt_queries else nq
     t1 = time.time()
     parts = []
-    for s0 in range(0, gt_nq, 2000):
-        parts.append(idx.brute_force(queries[s0:min(gt_nq, s0 + 2000)]))
+    for s0 in range(0, gt_nq, 10000):
+        parts.append(idx.brute_force(queries[s0:min(gt_nq, s0 + 10000)]))
         if n >= 5_000_000:
-            log(f"  ground truth {min(gt_nq, s0 + 2000):,} / {gt_nq:,} ({time.time() - t1:.1f}s)")
+            log(f"  ground truth {min(gt_nq, s0 + 10000):,} / {gt_nq:,} ({time.time() - t1:.1f}s)")
     gt_ids = np.concatenate([p_[0] for p_ in parts])
-    if base.dtype.kind == "f":  # FP64 re-rank of the FP32 candidates
+    if base.dtype.kind == "f":
+        # recheck: the FP64 GEMM form |x|^2 - 2 q.x + |q|^2 can differ from the
+        # direct FP64 sum in its last bits; the 32 candidates are re-ranked by
+        # the direct sum
         for i in range(gt_nq):
             c = gt_ids[i]
             A, Q = base[c].astype(np.float64), queries[i].astype(np.float64)
@@ -687,6 +690,10 @@ def main():
             h.timer_begin()
             run(ef)
             ms = h.timer_end()
+            flags = h.device_error()
+            if flags:
+                sys.exit(f"[{which}] ef={ef}: device reported search failure flags {flags} "
+                         "(1 visited table full, 2 bad arguments, 4 candidate heap full)")
             ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
             r = recall_at_k(ids, gt_ids, k)
             out.append([ef, round(r, 4), round(nq / (ms / 1e3), 1)])

@@ -218,10 +218,12 @@ int pk_last_launch(pk_index *h, int *grid, int *lds_bytes);
 
 /* Exhaustive nearest neighbours of nq host queries over the base rows set
  * with pk_index_set_base (rows [0, highest row set)): the 32 closest per
- * query, ascending by (distance, row id), to out_ids / out_dists (nq x 32).
- * u8 / i8 distances are exact; f16 / f32 are FP32 sums, to be re-ranked in
- * FP64 by the caller.  No reference counterpart on the hot path: the
- * reference reads precomputed ground-truth files
+ * query, ascending by (distance, row id), to out_ids / out_dists (nq x 32);
+ * -1 / +inf past the row count.  Dot products run on the matrix cores:
+ * u8 / i8 as int8 MFMA with exact int32 distances; f16 / f32 as FP64 MFMA
+ * (distances in FP64, returned rounded to FP32).  Rows up to 256 bytes
+ * (u8 / i8) or 512 bytes (f16 / f32).  No reference counterpart on the hot
+ * path: the reference reads precomputed ground-truth files
  * (benchmark/bench_util.hpp:196-202); this produces them. */
 int pk_brute_force(pk_index *h, const void *queries, int64_t nq, int32_t *out_ids,
                    float *out_dists);

@@ -402,7 +402,7 @@ class Index:
             progress(min(len(order), s + chunk), len(order))
 
     def brute_force(self, queries: np.ndarray):
-        """The BRUTE_K nearest base rows per query (exact for u8/i8; FP32 for floats)."""
+        """The BRUTE_K nearest base rows per query (exact int32 for u8/i8; FP64 for floats, returned as FP32)."""
         q = np.ascontiguousarray(queries, dtype=self.dtype)
         nq = q.shape[0]
         ids = np.empty((nq, BRUTE_K), np.int32)

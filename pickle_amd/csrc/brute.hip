@@ -1,200 +1,246 @@
-// brute.hip -- exhaustive k-nearest-neighbour search (ground truth).
+// brute.hip -- exhaustive k-nearest-neighbour search on the matrix cores.
 //
 // No reference counterpart on the hot path: the reference's benchmark reads
 // ground-truth files computed elsewhere (benchmark/bench_util.hpp:196-202,
-// benchmark/hnsw_pickle.cpp:94-112).  This is the brute-force side of the
-// recall measurement.
+// benchmark/hnsw_pickle.cpp:94-112).  This produces them.
 //
-// A workgroup owns 64 queries; its four waves stream disjoint quarters of
-// the base rows.  Each lane keeps one query in registers and its own sorted
-// top-KB list; a row address is the same for every lane of a wave, so each
-// row is fetched once per wave and broadcast.  The four partial lists of a
-// query are merged through LDS at the end.
-//   u8 / i8: exact int32 distances (v_dot4 on 16-byte chunks)
-//   f16 / f32: FP32 fused multiply-adds in element order; callers re-rank
-//   the KB candidates in FP64.
-// Ties rank by the lower row id.
+// The distance matrix is a dense contraction, so the dot products run as
+// MFMA tile products and only the top-k selection is vector work:
+//   u8 / i8: v_mfma_i32_32x32x32_i8 over 32 base rows x 32 queries.  u8 is
+//     moved to the signed range by x' = x - 128 (an XOR of the top bit);
+//     L2 is |q'|^2 + |x'|^2 - 2 q'.x' (the shift cancels) and -IP adds
+//     back 128 (sum q' + sum x') + 128^2 dim.  Every sum is exact in int32.
+//   f16 / f32: v_mfma_f64_16x16x4f64 over 16 base rows x 16 queries, on
+//     the values widened to FP64 (exactly); L2 is |x|^2 - 2 q.x + |q|^2 in
+//     FP64.
+// The order of K inside a tile does not change a dot product, so every lane
+// feeds one contiguous 16-byte chunk of its row to each operand; the A and
+// B operands pair the same chunk bytes lane group by lane group.  Staged
+// query rows are zero past the vector (after the u8 shift), so bytes past
+// `dim` in a base row contribute nothing.
+//
+// Selection: each lane owns one query and a fixed half (integer tiles) or
+// quarter (FP64 tiles) of every tile's rows, and keeps a sorted list of its
+// 32 best (key, row) pairs in registers.  Rows reach a lane in increasing
+// order, so a strict compare against the list's last key keeps the lower
+// row first among equal distances.  The grid splits the base rows as well
+// as the queries; k_brute_merge takes the 32 best entries of every query's
+// lists, ascending by (distance, row id).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
+#include <vector>
 
 #include "../../include/pickle_gpu.h"
-#include "distance.hpp"
+#include "brute.hpp"
 #include "index_impl.hpp"
 
 namespace pk {
 
-constexpr int kBruteQ = 64;
-constexpr int kBruteK = 32;  // candidates kept per query
+// ---- per-row constants -------------------------------------------------
 
-template <int KB>
-__device__ __forceinline__ void topk_insert(float (&bd)[KB], int (&bi)[KB], float d, int id) {
-    // lexicographic (distance, id) order; list sorted ascending
-    bool better = d < bd[KB - 1] || (d == bd[KB - 1] && id < bi[KB - 1]);
-    if (!better) return;
-    bd[KB - 1] = d;
-    bi[KB - 1] = id;
+// integer rows: |x'|^2 (L2) or -128 sum x' (u8 -IP); rows past nbase get 0
+template <int DT, int MET>
+__global__ __launch_bounds__(256) void k_brute_rowconst(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
+                                                        int dim, int64_t npad, int32_t* __restrict__ crow) {
+    constexpr uint32_t FLIP = DT == DT_U8 ? 0x80808080u : 0u;
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= npad) return;
+    int acc = 0;
+    if (r < nbase) {
+        const uint8_t* rp = base + r * (int64_t)vstride;
+        for (int c = 0; c * 16 < dim; c++) {
+            const uint4 u = ld16_global(rp + 16 * c);
+            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-    for (int j = KB - 1; j > 0; j--) {
-        bool sw = bd[j] < bd[j - 1] || (bd[j] == bd[j - 1] && bi[j] < bi[j - 1]);
-        float td = sw ? bd[j - 1] : bd[j];
-        int ti = sw ? bi[j - 1] : bi[j];
-        bd[j - 1] = sw ? bd[j] : bd[j - 1];
-        bi[j - 1] = sw ? bi[j] : bi[j - 1];
-        bd[j] = td;
-        bi[j] = ti;
+            for (int i = 0; i < 4; i++) {
+                const int nb = dim - (16 * c + 4 * i);  // bytes of this word inside the vector
+                const uint32_t m = nb >= 4 ? ~0u : nb <= 0 ? 0u : ((1u << (8 * nb)) - 1u);
+                const int x = (int)((w[i] ^ FLIP) & m);
+                acc = MET == MET_L2 ? __builtin_amdgcn_sdot4(x, x, acc, false)
+                                    : __builtin_amdgcn_sdot4(x, 0x01010101, acc, false);
+            }
+        }
+    }
+    crow[r] = MET == MET_L2 ? acc : -128 * acc;
+}
+
+// ---- integer tiles -------------------------------------------------------
+
+// Queries are staged NKB * 32 bytes apart (u8 padded with 0x80, i8 with 0).
+// Grid: x = query blocks of kIntQ, y = row splits of tiles_per_split tiles.
+// Lists: pd / pi [q][y * 2 + lane / 32][kBruteK].
+template <int DT, int MET, int NKB>
+__global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
+                                                    const int32_t* __restrict__ crow, const uint8_t* __restrict__ qs,
+                                                    int64_t nq, int dim, int64_t tiles_per_split, int nlists,
+                                                    double* __restrict__ pd, int32_t* __restrict__ pi) {
+    constexpr int KB = kBruteK;
+    constexpr int QST = NKB * 32;
+    constexpr uint32_t FLIP = DT == DT_U8 ? 0x80808080u : 0u;
+    constexpr int COEF = MET == MET_L2 ? 2 : 1;
+    constexpr bool CROW = !(DT == DT_I8 && MET == MET_IP);
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 5, col = lane & 31;
+    const int64_t q = (int64_t)blockIdx.x * kIntQ + (threadIdx.x >> 6) * 32 + col;
+    const int64_t qc = q < nq ? q : nq - 1;
+
+    v4i bq[NKB];
+    int part = 0;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+        bq[kb] = as_v4i(ld16_global(qs + qc * QST + kb * 32 + h * 16), FLIP);
+        const int w[4] = {bq[kb].x, bq[kb].y, bq[kb].z, bq[kb].w};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            part = MET == MET_L2 ? __builtin_amdgcn_sdot4(w[i], w[i], part, false)
+                                 : __builtin_amdgcn_sdot4(w[i], 0x01010101, part, false);
+    }
+    part += __shfl_xor(part, 32, kWave);
+    const int cq = MET == MET_L2 ? part : (DT == DT_U8 ? -128 * part - 16384 * dim : 0);
+
+    int coff[NKB];  // the lane's chunk offsets, kept inside the row
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) coff[kb] = kb * 32 + h * 16 < vstride ? kb * 32 + h * 16 : 0;
+
+    int kd[KB], ki[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        kd[j] = INT_MAX;
+        ki[j] = INT_MAX;
+    }
+    int thr = INT_MAX;
+
+    const int64_t ntiles = (nbase + 31) / 32;
+    const int64_t t0 = (int64_t)blockIdx.y * tiles_per_split;
+    const int64_t t1 = t0 + tiles_per_split < ntiles ? t0 + tiles_per_split : ntiles;
+
+    v4i an[NKB];
+    int4 cn[4];
+    auto load_tile = [&](int64_t t) {
+        int64_t row = t * 32 + col;
+        row = row < nbase ? row : nbase - 1;
+        const uint8_t* rp = base + row * (int64_t)vstride;
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++) an[kb] = as_v4i(ld16_global(rp + coff[kb]), FLIP);
+        if (CROW) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const uint4 u = ld16_global(crow + t * 32 + 8 * g + 4 * h);
+                cn[g] = make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
+            }
+        }
+    };
+    if (t0 < t1) load_tile(t0);
+    for (int64_t t = t0; t < t1; t++) {
+        v4i a[NKB];
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++) a[kb] = an[kb];
+        int4 cr[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++) cr[g] = cn[g];
+        if (t + 1 < t1) load_tile(t + 1);
+
+        v16i acc = {};
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kb], bq[kb], acc, 0, 0, 0);
+
+        // acc[r] is row t*32 + 8 (r / 4) + 4 h + r % 4 against query col
+        int key[16];
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int4 c4 = cr[r >> 2];
+            const int c = (r & 3) == 0 ? c4.x : (r & 3) == 1 ? c4.y : (r & 3) == 2 ? c4.z : c4.w;
+            key[r] = (CROW ? c : 0) - COEF * acc[r];
+            any |= key[r] < thr;
+        }
+        if (ballot(any)) {
+            // one copy of the insert: take key[0], then shift the rest down
+#pragma unroll 1
+            for (int r = 0; r < 16; r++) {
+                const int64_t m = t * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (key[0] < thr && m < nbase) {
+                    sorted_insert(kd, ki, key[0], (int)m);
+                    thr = kd[KB - 1];
+                }
+#pragma unroll
+                for (int i = 0; i < 15; i++) key[i] = key[i + 1];
+            }
+        }
+    }
+
+    if (q < nq) {
+        const size_t o = ((size_t)q * nlists + (size_t)blockIdx.y * 2 + h) * KB;
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            pd[o + j] = ki[j] == INT_MAX ? __builtin_inf() : (double)(kd[j] + cq);
+            pi[o + j] = ki[j];
+        }
     }
 }
 
-// CH = 16-byte chunks per row
-template <int DT, int MET, int CH>
-__global__ __launch_bounds__(256) void k_brute(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
-                                               const uint8_t* __restrict__ queries, int64_t nq, int qbytes, int nch,
-                                               int32_t* __restrict__ out_ids, float* __restrict__ out_d) {
-    __shared__ float sd[4][kBruteK][kBruteQ];
-    __shared__ int si[4][kBruteK][kBruteQ];
+// ---- merge -----------------------------------------------------------------
+
+// One wave per query: lane i holds the head of list i (nlists <= 64); each
+// round the wave takes the least (distance, row id) head and its lane moves
+// to the next entry.
+__global__ __launch_bounds__(256) void k_brute_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
+                                                     int nlists, int64_t nq, int32_t* __restrict__ out_ids,
+                                                     float* __restrict__ out_d) {
+    constexpr int KB = kBruteK;
     const int lane = threadIdx.x & 63;
-    const int sub = threadIdx.x >> 6;
-    const int64_t q = (int64_t)blockIdx.x * kBruteQ + lane;
-    const bool live = q < nq;
-    constexpr bool INT = DT == DT_U8 || DT == DT_I8;
-
-    uint4 qv[CH];
-    const uint8_t* qp = queries + (live ? q : 0) * (int64_t)qbytes;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const bool has = lane < nlists;
+    const size_t o = ((size_t)q * nlists + (has ? lane : 0)) * KB;
+    int p = 0;
+    double hd = has ? pd[o] : __builtin_inf();
+    int hi = has ? pi[o] : INT_MAX;
+    for (int j = 0; j < KB; j++) {
+        double bd = hd;
+        int bi = hi, bl = lane;
 #pragma unroll
-    for (int c = 0; c < CH; c++) {
-        uint32_t w[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            uint32_t x = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                int off = c * 16 + u * 4 + b;
-                uint32_t by = off < qbytes ? qp[off] : 0;
-                x |= by << (8 * b);
-            }
-            w[u] = x;
+        for (int s = 32; s >= 1; s >>= 1) {
+            const double od = __shfl_xor(bd, s, kWave);
+            const int oi = __shfl_xor(bi, s, kWave);
+            const int ol = __shfl_xor(bl, s, kWave);
+            const bool take = od < bd || (od == bd && (oi < bi || (oi == bi && ol < bl)));
+            bd = take ? od : bd;
+            bi = take ? oi : bi;
+            bl = take ? ol : bl;
         }
-        qv[c] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    int qq = 0;
-    if (INT) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int c = 0; c < CH; c++) {
-            s = dot4(DT == DT_I8, qv[c].x, qv[c].x, s);
-            s = dot4(DT == DT_I8, qv[c].y, qv[c].y, s);
-            s = dot4(DT == DT_I8, qv[c].z, qv[c].z, s);
-            s = dot4(DT == DT_I8, qv[c].w, qv[c].w, s);
+        if (lane == 0) {
+            out_ids[q * KB + j] = bi == INT_MAX ? -1 : bi;
+            out_d[q * KB + j] = bi == INT_MAX ? __builtin_inff() : (float)bd;
         }
-        qq = (int)s;
-    }
-
-    float bd[kBruteK];
-    int bi[kBruteK];
-#pragma unroll
-    for (int j = 0; j < kBruteK; j++) {
-        bd[j] = __builtin_inff();
-        bi[j] = 0x7fffffff;
-    }
-
-    for (int64_t r = sub; r < nbase; r += 4) {
-        const uint4* row = reinterpret_cast<const uint4*>(base + r * (int64_t)vstride);
-        float d;
-        if (INT) {
-            uint32_t ab = 0, bb = 0;
-#pragma unroll
-            for (int c = 0; c < CH; c++) {
-                uint4 b = c < nch ? row[c] : make_uint4(0, 0, 0, 0);
-                ab = dot4(DT == DT_I8, qv[c].x, b.x, ab);
-                ab = dot4(DT == DT_I8, qv[c].y, b.y, ab);
-                ab = dot4(DT == DT_I8, qv[c].z, b.z, ab);
-                ab = dot4(DT == DT_I8, qv[c].w, b.w, ab);
-                if (MET == MET_L2) {
-                    bb = dot4(DT == DT_I8, b.x, b.x, bb);
-                    bb = dot4(DT == DT_I8, b.y, b.y, bb);
-                    bb = dot4(DT == DT_I8, b.z, b.z, bb);
-                    bb = dot4(DT == DT_I8, b.w, b.w, bb);
-                }
-            }
-            d = MET == MET_L2 ? (float)(qq + (int)bb - 2 * (int)ab) : -(float)(int)ab;
-        } else {
-            float acc = 0.0f;
-#pragma unroll
-            for (int c = 0; c < CH; c++) {
-                uint4 b = c < nch ? row[c] : make_uint4(0, 0, 0, 0);
-                uint32_t bw[4] = {b.x, b.y, b.z, b.w};
-                uint32_t aw[4] = {qv[c].x, qv[c].y, qv[c].z, qv[c].w};
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (DT == DT_F32) {
-                        float x = __uint_as_float(aw[u]), y = __uint_as_float(bw[u]);
-                        acc = MET == MET_L2 ? __builtin_fmaf(x - y, x - y, acc) : __builtin_fmaf(x, y, acc);
-                    } else {
-#pragma unroll
-                        for (int hh = 0; hh < 2; hh++) {
-                            _Float16 xa, ya;
-                            uint16_t xs = (uint16_t)(aw[u] >> (16 * hh)), ys = (uint16_t)(bw[u] >> (16 * hh));
-                            __builtin_memcpy(&xa, &xs, 2);
-                            __builtin_memcpy(&ya, &ys, 2);
-                            float x = (float)xa, y = (float)ya;
-                            acc = MET == MET_L2 ? __builtin_fmaf(x - y, x - y, acc) : __builtin_fmaf(x, y, acc);
-                        }
-                    }
-                }
-            }
-            d = MET == MET_L2 ? acc : -acc;
-        }
-        topk_insert(bd, bi, d, (int)r);
-    }
-
-#pragma unroll
-    for (int j = 0; j < kBruteK; j++) {
-        sd[sub][j][lane] = bd[j];
-        si[sub][j][lane] = bi[j];
-    }
-    __syncthreads();
-    if (sub == 0 && live) {
-        int pos[4] = {0, 0, 0, 0};
-        for (int j = 0; j < kBruteK; j++) {
-            int best = 0;
-            float bdv = __builtin_inff();
-            int biv = 0x7fffffff;
-            for (int s = 0; s < 4; s++) {
-                if (pos[s] >= kBruteK) continue;
-                float dv = sd[s][pos[s]][lane];
-                int iv = si[s][pos[s]][lane];
-                if (dv < bdv || (dv == bdv && iv < biv)) {
-                    bdv = dv;
-                    biv = iv;
-                    best = s;
-                }
-            }
-            pos[best]++;
-            out_d[q * kBruteK + j] = bdv;
-            out_ids[q * kBruteK + j] = biv == 0x7fffffff ? -1 : biv;
+        if (lane == bl) {
+            p++;
+            hd = p < KB ? pd[o + p] : __builtin_inf();
+            hi = p < KB ? pi[o + p] : INT_MAX;
         }
     }
 }
 
 template <int DT, int MET>
-static hipError_t brute_dt(int chunks, dim3 grid, hipStream_t s, const uint8_t* base, int64_t nbase, int vstride,
-                           const uint8_t* q, int64_t nq, int qbytes, int nch, int32_t* ids, float* d) {
-#define PK_BRUTE_CASE(CHV)                                                                                       \
-    if (chunks <= CHV) {                                                                                        \
-        hipLaunchKernelGGL((k_brute<DT, MET, CHV>), grid, dim3(256), 0, s, base, nbase, vstride, q, nq, qbytes, \
-                           nch, ids, d);                                                                        \
-        return hipGetLastError();                                                                               \
+static hipError_t launch_int(int nkb, const BrutePlan& p, hipStream_t s, const uint8_t* base, int64_t nbase,
+                             int vstride, const int32_t* crow, const uint8_t* dq, int64_t nq, int dim, double* pd,
+                             int32_t* pi) {
+    dim3 grid((unsigned)p.qblocks, (unsigned)p.splits);
+#define PK_IMMA_CASE(N)                                                                                       \
+    if (nkb <= N) {                                                                                           \
+        hipLaunchKernelGGL((k_brute_imma<DT, MET, N>), grid, dim3(256), 0, s, base, nbase, vstride, crow, dq, \
+                           nq, dim, p.tiles_per_split, p.nlists, pd, pi);                                    \
+        return hipGetLastError();                                                                             \
     }
-    PK_BRUTE_CASE(2)
-    PK_BRUTE_CASE(4)
-    PK_BRUTE_CASE(6)
-    PK_BRUTE_CASE(8)
-    if (DT == DT_F32 || DT == DT_F16) {
-        PK_BRUTE_CASE(16)
-        PK_BRUTE_CASE(32)
-    }
-#undef PK_BRUTE_CASE
+    PK_IMMA_CASE(1)
+    PK_IMMA_CASE(2)
+    PK_IMMA_CASE(3)
+    PK_IMMA_CASE(4)
+    PK_IMMA_CASE(6)
+    PK_IMMA_CASE(8)
+#undef PK_IMMA_CASE
     return hipErrorInvalidValue;
 }
 
@@ -206,40 +252,78 @@ extern "C" int pk_brute_force(pk_index* h, const void* queries, int64_t nq, int3
     if (!h || (nq > 0 && (!queries || !out_ids || !out_dists))) return set_error(PK_ERR_ARG, "null argument");
     if (!h->base || h->base_rows < 1) return set_error(PK_ERR_STATE, "no base rows");
     if (nq == 0) return PK_OK;
-    int chunks = h->vstride / 16;
-    bool is_int = h->dtype == PK_UINT8 || h->dtype == PK_INT8;
-    if ((is_int && chunks > 8) || chunks > 32) return set_error(PK_ERR_ARG, "row too wide for the brute-force kernel");
+    const bool is_int = h->dtype == PK_UINT8 || h->dtype == PK_INT8;
+    const int dim = h->dim;
+    const int64_t nbase = h->base_rows;
+    // staged query stride: 32-byte K blocks (integer) or 64-byte super-chunks (float)
+    const int unit = is_int ? 32 : 64;
+    const int nunits = (h->qbytes + unit - 1) / unit;
+    if (nunits > 8) return set_error(PK_ERR_ARG, "row too wide for the brute-force kernel (at most 256 bytes of u8/i8, 512 of f16/f32)");
+    static const int kRound[9] = {1, 1, 2, 3, 4, 6, 6, 8, 8};
+    const int nu = kRound[nunits];
+    const int qst = nu * unit;
+    const BrutePlan p = is_int ? plan_brute(nq, nbase, kIntQ, 32, 2) : plan_brute(nq, nbase, kFltQ, 16, 4);
+    const int64_t npad = (nbase + 31) / 32 * 32;
+
+    std::vector<uint8_t> staged((size_t)nq * qst, h->dtype == PK_UINT8 ? 0x80 : 0x00);
+    for (int64_t i = 0; i < nq; i++)
+        std::memcpy(staged.data() + (size_t)i * qst, (const uint8_t*)queries + (size_t)i * h->qbytes, h->qbytes);
+
     uint8_t* dq = nullptr;
+    void* rc = nullptr;  // crow (int32) or row norms (FP64)
+    double* pd = nullptr;
+    int32_t* pi = nullptr;
     int32_t* di = nullptr;
     float* dd = nullptr;
     auto cleanup = [&]() {
-        if (dq) (void)hipFree(dq);
-        if (di) (void)hipFree(di);
-        if (dd) (void)hipFree(dd);
+        for (void* x : {(void*)dq, rc, (void*)pd, (void*)pi, (void*)di, (void*)dd})
+            if (x) (void)hipFree(x);
     };
-    if (hipMalloc(&dq, (size_t)nq * h->qbytes) != hipSuccess || hipMalloc(&di, (size_t)nq * kBruteK * 4) != hipSuccess ||
-        hipMalloc(&dd, (size_t)nq * kBruteK * 4) != hipSuccess) {
+    const size_t nl = (size_t)nq * p.nlists * kBruteK;
+    if (hipMalloc(&dq, staged.size()) != hipSuccess || hipMalloc(&rc, (size_t)npad * 8) != hipSuccess ||
+        hipMalloc(&pd, nl * 8) != hipSuccess || hipMalloc(&pi, nl * 4) != hipSuccess ||
+        hipMalloc(&di, (size_t)nq * kBruteK * 4) != hipSuccess || hipMalloc(&dd, (size_t)nq * kBruteK * 4) != hipSuccess) {
         cleanup();
         return set_error(PK_ERR_NOMEM, "brute force: device allocation failed");
     }
-    hipError_t e = hipMemcpyAsync(dq, queries, (size_t)nq * h->qbytes, hipMemcpyHostToDevice, h->stream);
-    dim3 grid((unsigned)((nq + kBruteQ - 1) / kBruteQ));
+    hipStream_t s = h->stream;
+    hipError_t e = hipMemcpyAsync(dq, staged.data(), staged.size(), hipMemcpyHostToDevice, s);
+    const dim3 rgrid((unsigned)(npad / 256 + 1));
+    const uint8_t* b = h->base;
+    const int vs = h->vstride;
     if (e == hipSuccess) {
         switch (h->dtype * 4 + h->metric) {
-            case DT_U8 * 4 + MET_L2: e = brute_dt<DT_U8, MET_L2>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_U8 * 4 + MET_IP: e = brute_dt<DT_U8, MET_IP>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_I8 * 4 + MET_L2: e = brute_dt<DT_I8, MET_L2>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_I8 * 4 + MET_IP: e = brute_dt<DT_I8, MET_IP>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_F32 * 4 + MET_L2: e = brute_dt<DT_F32, MET_L2>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_F32 * 4 + MET_IP: e = brute_dt<DT_F32, MET_IP>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_F16 * 4 + MET_L2: e = brute_dt<DT_F16, MET_L2>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
-            case DT_F16 * 4 + MET_IP: e = brute_dt<DT_F16, MET_IP>(chunks, grid, h->stream, h->base, h->base_rows, h->vstride, dq, nq, h->qbytes, chunks, di, dd); break;
+            case DT_U8 * 4 + MET_L2:
+                hipLaunchKernelGGL((k_brute_rowconst<DT_U8, MET_L2>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
+                e = launch_int<DT_U8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                break;
+            case DT_U8 * 4 + MET_IP:
+                hipLaunchKernelGGL((k_brute_rowconst<DT_U8, MET_IP>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
+                e = launch_int<DT_U8, MET_IP>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                break;
+            case DT_I8 * 4 + MET_L2:
+                hipLaunchKernelGGL((k_brute_rowconst<DT_I8, MET_L2>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
+                e = launch_int<DT_I8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                break;
+            case DT_I8 * 4 + MET_IP:
+                e = launch_int<DT_I8, MET_IP>(nu, p, s, b, nbase, vs, nullptr, dq, nq, dim, pd, pi);
+                break;
+            case DT_F32 * 4 + MET_L2:
+            case DT_F32 * 4 + MET_IP:
+            case DT_F16 * 4 + MET_L2:
+            case DT_F16 * 4 + MET_IP:
+                e = launch_brute_fp64(h->dtype, h->metric, nu, p, s, b, nbase, vs, dim, npad, (double*)rc, dq, nq, pd, pi);
+                break;
             default: e = hipErrorInvalidValue;
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out_ids, di, (size_t)nq * kBruteK * 4, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(out_dists, dd, (size_t)nq * kBruteK * 4, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_brute_merge, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, pd, pi, p.nlists, nq, di, dd);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out_ids, di, (size_t)nq * kBruteK * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_dists, dd, (size_t)nq * kBruteK * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("brute force: ") + hipGetErrorString(e));
     return PK_OK;

@@ -1,0 +1,105 @@
+// brute.hpp -- shared parts of the exhaustive k-nearest-neighbour kernels
+// (brute.hip: integer tiles, merge and the entry point; brute_fp64.hip:
+// FP64 tiles for f16 / f32 rows).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "distance.hpp"
+
+namespace pk {
+
+constexpr int kBruteK = 32;    // candidates returned per query
+constexpr int kIntQ = 128;     // queries per workgroup, integer tiles (4 waves x 32)
+constexpr int kFltQ = 64;      // queries per workgroup, FP64 tiles (4 waves x 16)
+constexpr int kMaxLists = 64;  // lists per query that one merge wave takes
+#ifndef PK_BRUTE_WGS
+#define PK_BRUTE_WGS 512
+#endif
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Branch-free selects: nested ?: chains over unrolled register arrays come
+// out of the front end as branches, which stay branches after optimisation.
+__device__ __forceinline__ int sel_bits(bool c, int a, int b) {
+    const int m = -(int)c;
+    return (a & m) | (b & ~m);
+}
+__device__ __forceinline__ double sel_bits(bool c, double a, double b) {
+    const long long m = -(long long)c;
+    return __longlong_as_double((__double_as_longlong(a) & m) | (__double_as_longlong(b) & ~m));
+}
+
+// Sorted ascending insert of (d, id), dropping the last entry.  The caller
+// only inserts ids larger than every id in the list, so the new entry goes
+// after any equal keys and the list stays sorted by (key, id).  Written as
+// a shift with no dependence between positions: entry j keeps its value
+// while it is <= d, the first larger position takes d, and the rest take
+// their predecessor.
+template <typename K, int KB>
+__device__ __forceinline__ void sorted_insert(K (&kd)[KB], int (&ki)[KB], K d, int id) {
+    bool le[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) le[j] = kd[j] <= d;
+#pragma unroll
+    for (int j = KB - 1; j > 0; j--) {
+        kd[j] = sel_bits(le[j], kd[j], sel_bits(le[j - 1], d, kd[j - 1]));
+        ki[j] = sel_bits(le[j], ki[j], sel_bits(le[j - 1], id, ki[j - 1]));
+    }
+    kd[0] = sel_bits(le[0], kd[0], d);
+    ki[0] = sel_bits(le[0], ki[0], id);
+}
+
+__device__ __forceinline__ v4i as_v4i(uint4 u, uint32_t flip) {
+    v4i v;
+    v.x = (int)(u.x ^ flip);
+    v.y = (int)(u.y ^ flip);
+    v.z = (int)(u.z ^ flip);
+    v.w = (int)(u.w ^ flip);
+    return v;
+}
+
+// element j of a 16-byte chunk of f32 (4) or f16 (8) values, widened to FP64
+template <int DT>
+__device__ __forceinline__ double chunk_elem(const uint4& u, int j) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    if (DT == DT_F32) return (double)__uint_as_float(w[j]);
+    const uint16_t bits = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
+    _Float16 x;
+    __builtin_memcpy(&x, &bits, 2);
+    return (double)(float)x;
+}
+
+struct BrutePlan {
+    int64_t qblocks, splits, tiles_per_split;
+    int nlists;
+};
+
+inline BrutePlan plan_brute(int64_t nq, int64_t nbase, int q_per_block, int rows_per_tile, int lists_per_split) {
+    BrutePlan p;
+    p.qblocks = (nq + q_per_block - 1) / q_per_block;
+    const int64_t ntiles = (nbase + rows_per_tile - 1) / rows_per_tile;
+    // Every (query, split) pair builds its own lists, and the inserts that
+    // cost grow with the splits, so split only as far as filling the chip
+    // needs: PK_BRUTE_WGS workgroups of 4 waves.
+    int64_t s = (PK_BRUTE_WGS + p.qblocks - 1) / p.qblocks;
+    const int64_t smax = kMaxLists / lists_per_split;
+    s = s < smax ? s : smax;
+    s = s < ntiles ? s : ntiles;
+    s = s > 1 ? s : 1;
+    p.tiles_per_split = (ntiles + s - 1) / s;
+    p.splits = (ntiles + p.tiles_per_split - 1) / p.tiles_per_split;
+    p.nlists = (int)(p.splits * lists_per_split);
+    return p;
+}
+
+// FP64 tiles (brute_fp64.hip): row norms when L2, then the tile kernel
+hipError_t launch_brute_fp64(int dtype, int metric, int nsc, const BrutePlan& p, hipStream_t s,
+                             const uint8_t* base, int64_t nbase, int vstride, int dim, int64_t npad,
+                             double* nrm, const uint8_t* dq, int64_t nq, double* pd, int32_t* pi);
+
+}  // namespace pk

@@ -445,7 +445,10 @@ static int ensure_scratch(pk_index* h, SearchCtx& c, int waves, int vbits) {
     c.vbits = 0;
     int rc;
     size_t slots = (size_t)waves << vbits;
-    if ((rc = dalloc(&c.cglob, (size_t)waves * h->cg_cap))) return rc;
+    // the candidate heap only holds first-visited ids, so with room for
+    // 2^vbits entries it cannot fill before the visited table does
+    const int cg_cap = std::max(h->cg_cap, 1 << vbits);
+    if ((rc = dalloc(&c.cglob, (size_t)waves * cg_cap))) return rc;
     if ((rc = dalloc(&c.vtab, slots))) return rc;
     if ((rc = dalloc(&c.vlog, slots))) return rc;
     HIPCHK(hipMemsetAsync(c.vtab, 0, slots * sizeof(uint32_t), c.stream));
@@ -453,6 +456,7 @@ static int ensure_scratch(pk_index* h, SearchCtx& c, int waves, int vbits) {
     if (!c.err && (rc = dalloc(&c.err, 1))) return rc;
     c.waves = waves;
     c.vbits = vbits;
+    c.cg_cap = cg_cap;
     return PK_OK;
 }
 }  // namespace pk
@@ -527,7 +531,7 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     a.cglob = c.cglob;
     a.vtab = c.vtab;
     a.vlog = c.vlog;
-    a.cg_cap = h->cg_cap;
+    a.cg_cap = c.cg_cap;
     a.vg_bits = vbits;
     a.wcap = wcap;
     a.ccap = ccap;

@@ -63,6 +63,7 @@ struct SearchCtx {
     int32_t* vlog = nullptr;
     int waves = 0;
     int vbits = 0;
+    int cg_cap = 0;  // candidate-heap entries per wave in device memory (>= 2^vbits)
     int32_t* qcounter = nullptr;
     int32_t* err = nullptr;
 };

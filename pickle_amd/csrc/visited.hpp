@@ -112,8 +112,21 @@ struct WaveVisited {
     // Empties the table (the slots this search filled) and orders the clears
     // before the next search's atomics.
     __device__ void clear(int lane) {
-        // log entries were written by other lanes: read them from L2
-        for (int i = lane; i < count; i += kWave) tab[__hip_atomic_load(&log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = 0u;
+        // log entries were written by other lanes: read them from L2.  Eight
+        // reads go out before their eight stores, so the clear costs a few
+        // round trips rather than one per 64 slots.
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < count; i0 += kWave * U) {
+            int slot[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * kWave + lane;
+                slot[u] = i < count ? __hip_atomic_load(&log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (slot[u] >= 0) tab[slot[u]] = 0u;
+        }
         peak = count > peak ? count : peak;
         count = 0;
         __builtin_amdgcn_s_waitcnt(0);  // stores reach L2 before the next atomics

@@ -519,7 +519,7 @@ def main():
     ap.add_argument("--gt-queries", type=int, default=0, help="queries with ground truth (0 = all)")
     ap.add_argument("--build", choices=["cpu", "gpu"], default=None,
                     help="graph from the reference's CPU build or the GPU batched build (default per workload)")
-    ap.add_argument("--insert-batch", type=int, default=4096)
+    ap.add_argument("--insert-batch", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", choices=["none", "f16", "i8"], default=None,
                     help="search a low-precision traversal copy + FP32 re-rank (float workloads)")
@@ -679,18 +679,28 @@ def main():
         return tidx, (lambda ef, c=False: tidx.search_rerank_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt,
                                                                     d_ctr if c else None))
 
+    vbits = {}
+
     def sweep(which, target, efs):
         h, run = backend(which)
         out, chosen = [], None
         for ef in efs:
             if ef < k:
                 continue
-            run(ef)
-            h.sync()
-            h.timer_begin()
-            run(ef)
-            ms = h.timer_end()
-            flags = h.device_error()
+            while True:
+                run(ef)
+                h.sync()
+                h.timer_begin()
+                run(ef)
+                ms = h.timer_end()
+                flags = h.device_error()
+                if flags == 1 and vbits.get(which, 16) < 20:
+                    # a query filled its visited table: a larger one per wave
+                    vbits[which] = vbits.get(which, 16) + 1
+                    h.set_tuning(0, vbits[which], 0)
+                    log(f"[{which}] ef={ef}: visited table full; retrying with 2^{vbits[which]} slots per wave")
+                    continue
+                break
             if flags:
                 sys.exit(f"[{which}] ef={ef}: device reported search failure flags {flags} "
                          "(1 visited table full, 2 bad arguments, 4 candidate heap full)")
@@ -715,7 +725,7 @@ def main():
         if args.no_autotune:
             return
         h, _ = backend(which)
-        ns = min(nq, 4096)
+        ns = min(nq, 16384)
         if which == "fp32":
             dq, hq = d_q, None
         else:
@@ -731,8 +741,27 @@ def main():
             hq = abi.DeviceBuffer.from_array(np.ascontiguousarray(qc))
             dq = hq
         t = h.autotune(dq, ns, k, ef)
-        tuned[which] = t
-        log(f"[{which}] autotuned at ef={ef}: {t}")
+        # the tuner times a sample of the batch with k outputs; keep its choice
+        # only if the whole batch, as this leg runs it, is faster with it
+        _, run = backend(which)
+
+        def batch_ms():
+            run(ef)
+            h.sync()
+            h.timer_begin()
+            run(ef)
+            return h.timer_end()
+
+        tuned_ms = batch_ms()
+        h.set_tuning(0, vbits.get(which, 0), 0)
+        base_ms = batch_ms()
+        if tuned_ms < base_ms:
+            h.set_tuning(t["ccap"], t["vbits"], t["blocks_per_cu"])
+            tuned[which] = dict(t, kept=True)
+        else:
+            tuned[which] = dict(t, kept=False)
+        log(f"[{which}] autotuned at ef={ef}: {t}; whole batch {tuned_ms:.2f} ms tuned vs {base_ms:.2f} ms "
+            f"default -> {'tuned' if tuned_ms < base_ms else 'default'}")
 
     def timed(which, ef):
         h, run = backend(which)

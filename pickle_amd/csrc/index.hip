@@ -283,9 +283,7 @@ int pk_index_set_base(pk_index* h, const void* rows, int64_t first, int64_t n) {
         if (rc) return rc;
         HIPCHK(hipMemsetAsync(h->base, 0, (size_t)h->capacity * h->vstride, h->stream));
     }
-    if (n > 0)
-        HIPCHK(hipMemcpy2DAsync(h->base + (size_t)first * h->vstride, h->vstride, rows, h->qbytes, h->qbytes, n,
-                                hipMemcpyHostToDevice, h->stream));
+    if (n > 0) HIPCHK(upload_rows(h->base + (size_t)first * h->vstride, h->vstride, rows, h->qbytes, n, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->base_rows = std::max(h->base_rows, first + n);
     return PK_OK;
@@ -294,6 +292,29 @@ int pk_index_set_base(pk_index* h, const void* rows, int64_t first, int64_t n) {
 }  // extern "C"
 
 namespace pk {
+// A pitched copy from pageable host memory goes row by row (microseconds per
+// row: over a minute for 10M rows), so rows are copied flat when the pitches
+// match and otherwise padded on the host in 64 MB pieces.
+hipError_t upload_rows(uint8_t* dst, size_t dst_pitch, const void* src, size_t row_bytes, int64_t n, hipStream_t s) {
+    const uint8_t* p = static_cast<const uint8_t*>(src);
+    if (row_bytes == dst_pitch) {
+        hipError_t e = hipMemcpyAsync(dst, p, (size_t)n * row_bytes, hipMemcpyHostToDevice, s);
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)((64u << 20) / dst_pitch));
+    std::vector<uint8_t> tmp;
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        const int64_t m = std::min(chunk, n - r0);
+        tmp.assign((size_t)m * dst_pitch, 0);  // padding bytes are zero
+        for (int64_t i = 0; i < m; i++)
+            std::memcpy(tmp.data() + (size_t)i * dst_pitch, p + (size_t)(r0 + i) * row_bytes, row_bytes);
+        hipError_t e = hipMemcpyAsync(dst + (size_t)r0 * dst_pitch, tmp.data(), tmp.size(), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // the layer's own vector copy: base row ext[i] where has[i], zeros elsewhere
 int gather_layer_vectors(pk_index* h, int level) {
     Layer& L = h->layers[level];
@@ -473,6 +494,16 @@ struct SearchPlan {
 };
 
 // validation, LDS plan and scratch (the part that may allocate)
+// Visited-table size per wave (2^bits slots): room for the ids an ef-wide
+// beam visits on large graphs (hard queries on 10M rows visit over 2^15 at
+// ef 128), and never more than twice the capacity, which cannot fill.
+static int default_vbits(int ef_max, int64_t capacity) {
+    int v = ef_max <= 32 ? 14 : (ef_max <= 64 ? 15 : (ef_max <= 256 ? 16 : (ef_max <= 512 ? 17 : 18)));
+    int cap_bits = 10;
+    while (cap_bits < 20 && ((int64_t)1 << cap_bits) < 2 * capacity) cap_bits++;
+    return std::min(v, cap_bits);
+}
+
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
@@ -514,7 +545,7 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     int wcap = result_capacity_for(ef_max);
     if (wcap < 0) return fail(PK_ERR_ARG, "ef_max %d above the largest supported ef (630)", ef_max);
     int ccap = t.ccap > 0 ? t.ccap : (ef_max <= 64 ? 512 : 1024);
-    int vbits = t.vbits > 0 ? t.vbits : (ef_max <= 32 ? 14 : (ef_max <= 128 ? 15 : (ef_max <= 512 ? 16 : 17)));
+    int vbits = t.vbits > 0 ? t.vbits : default_vbits(ef_max, h->capacity);
     if (h->capacity > kVisitMaxId) return fail(PK_ERR_CAPACITY, "search supports at most %lld ids", (long long)kVisitMaxId);
     size_t per_wave = (size_t)wcap * 8 + (size_t)ccap * 8 + 64 * 4 * 2 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
@@ -655,6 +686,11 @@ int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k
         HIPCHK(hipEventSynchronize(h->ev1));
         HIPCHK(hipEventElapsedTime(ms, h->ev0, h->ev1));
         *ms /= 2;
+        // a configuration whose searches fail (full visited table or
+        // candidate heap) ends those queries early: never a candidate
+        int32_t flags = 0;
+        HIPCHK(hipMemcpy(&flags, h->sc.err, sizeof(flags), hipMemcpyDeviceToHost));
+        if (flags) return fail(PK_ERR_DEVICE, "search failed on the device (flags %d)", flags);
         return PK_OK;
     };
     // coordinate descent: resident workgroups per CU, then the candidate
@@ -662,7 +698,16 @@ int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k
     // not fit are skipped)
     SearchTuning best = saved;
     float best_ms = 0;
-    if ((rc = time_one(best, &best_ms))) {
+    // the starting point must complete every query: grow its visited table
+    // until it does
+    for (;;) {
+        rc = time_one(best, &best_ms);
+        if (rc != PK_ERR_DEVICE) break;
+        const int v = best.vbits > 0 ? best.vbits : default_vbits(ef, h->capacity);
+        if (v >= 20) break;
+        best.vbits = v + 1;
+    }
+    if (rc) {
         h->tuning = saved;
         dfree(buf);
         return rc;
@@ -692,8 +737,9 @@ int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k
         t.ccap = c;
         try_set(t);
     }
-    const int vb0 = ef <= 32 ? 14 : (ef <= 128 ? 15 : (ef <= 512 ? 16 : 17));
+    const int vb0 = best.vbits > 0 ? best.vbits : default_vbits(ef, h->capacity);
     for (int v : {vb0 - 1, vb0 + 1}) {
+        if (v < 8 || v > 20) continue;
         SearchTuning t = best;
         t.vbits = v;
         try_set(t);

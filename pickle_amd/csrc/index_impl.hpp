@@ -87,6 +87,10 @@ int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int wa
 int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
                         int maxl);
 int gather_layer_vectors(pk_index* h, int level);
+// n host rows of row_bytes each, to device rows dst_pitch apart (padding
+// bytes zero), in stream order; returns when the host memory is free
+hipError_t upload_rows(uint8_t* dst, size_t dst_pitch, const void* src, size_t row_bytes, int64_t n,
+                       hipStream_t s);
 // one batched search on a given context (its stream and scratch); the
 // handle's own calls use h->sc
 int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,

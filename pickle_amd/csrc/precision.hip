@@ -279,8 +279,7 @@ int pk_index_set_full_rows(pk_index* h, const float* rows, int64_t first, int64_
     }
     if (n > 0) {
         uint8_t* dst = h->full + (size_t)first * h->full_vstride;
-        PCHK(hipMemcpy2DAsync(dst, h->full_vstride, rows, (size_t)h->dim * 4, (size_t)h->dim * 4, n,
-                              hipMemcpyHostToDevice, h->stream));
+        PCHK(upload_rows(dst, h->full_vstride, rows, (size_t)h->dim * 4, n, h->stream));
         if ((rc = quantize_rows(h, reinterpret_cast<const float*>(dst), h->full_vstride / 4, n,
                                 h->base + (size_t)first * h->vstride, h->vstride)))
             return rc;

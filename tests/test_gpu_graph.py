@@ -67,10 +67,12 @@ def test_graph_replays_match_eager(u8_index):
     idx.sync()
     np.testing.assert_array_equal(di.to_array(np.int32, (nq, 32)), want2[0])
     # while the graph lives, calls that would move its pointers are refused
+    idx.set_tuning(0, 20, 0)  # a visited table larger than the graph's scratch
     with pytest.raises(abi.PickleError):
-        idx.search(queries, 10, 600)  # needs a bigger visited table
+        idx.search(queries, 10, 40)
     graph.close()
-    idx.search(queries, 10, 600)
+    idx.search(queries, 10, 40)
+    idx.set_tuning(0, 0, 0)
 
 
 def test_autotune_keeps_results(u8_index):

@@ -161,8 +161,9 @@ def _csv_rows(directory, suffix):
 
 
 def _short_kernel(name):
-    for key in ("k_search", "k_brute", "k_copy16", "k_gather_rows", "k_fill_i32", "k_insert_search",
-                "k_insert_rows", "k_insert_edges", "k_rerank"):
+    for key in ("k_search", "k_brute_imma", "k_brute_dmma", "k_brute_rowconst", "k_brute_rownorm", "k_brute_merge",
+                "k_copy16", "k_gather_rows", "k_fill_i32", "k_insert_search", "k_insert_rows", "k_insert_edges",
+                "k_rerank"):
         if key in name:
             return key + (name[name.index("<"):name.index(">") + 1] if "<" in name and key != "k_copy16" else "")
     return name[:60]
@@ -656,7 +657,15 @@ def main():
             A, Q = base[c].astype(np.float64), queries[i].astype(np.float64)
             d64 = ((A - Q) ** 2).sum(1) if metric == "l2" else -(A @ Q)
             gt_ids[i] = c[np.lexsort((c, d64))]
-    log(f"ground truth (GPU brute force, {gt_nq} queries) in {time.time() - t1:.1f}s")
+    gt_s = time.time() - t1
+    log(f"ground truth (GPU brute force, {gt_nq} queries) in {gt_s:.1f}s")
+    # the brute force is a dense contraction on the matrix cores: int8 MFMA
+    # for u8/i8 rows, FP64 MFMA for f16/f32 (wall time, host staging included)
+    gt_ops = 2.0 * gt_nq * n * dim
+    ground_truth = {"queries": gt_nq, "seconds": round(gt_s, 4),
+                    "kernel": "k_brute_imma" if base.dtype.kind in "ui" else "k_brute_dmma",
+                    "tera_ops_per_s": round(gt_ops / gt_s / 1e12, 2),
+                    "mfma_peak_tera_ops_per_s": 5000.0 if base.dtype.kind in "ui" else 78.6}
 
     # HBM-resident buffers for the timed runs
     d_q = abi.DeviceBuffer.from_array(queries)
@@ -895,6 +904,7 @@ def main():
                      "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),
                                    "dists": round(float(dists.mean()), 2)}},
         "cpu_baseline": cpu,
+        "ground_truth": ground_truth,
     }
     print(json.dumps(line), flush=True)
 

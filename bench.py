@@ -260,11 +260,13 @@ def summarize(args):
 
 
 def run_serving(args):
-    """Open-loop serving run: Poisson arrivals of query batches (sizes
+    """Open-loop serving runs: Poisson arrivals of query batches (sizes
     log-uniform in [1, 4096], ef uniform in [16, 256], k in {10, 100}) mixed
-    with insert batches of 1,024 new vectors at --insert-share of requests,
-    through the library's serving loop on a GPU-built index.  Reports p50 /
-    p99 request latency, completed queries per second and the loop's batching."""
+    with insert batches of 1,024 new vectors at each --insert-share (a comma
+    list; the mixes run one after another on one GPU-built index, inserts
+    accumulating), through the library's serving loop.  Reports p50 / p99
+    request latency, completed queries per second and the loop's batching
+    for every mix, in one JSON line (the first mix is its value)."""
     import threading
 
     from pickle_amd import abi, datasets
@@ -272,8 +274,10 @@ def run_serving(args):
     wl = WORKLOADS[args.workload]
     n = args.n or wl["n"]
     n_req = args.requests
-    n_ins = max(1, int(round(n_req * args.insert_share)))
-    extra = n_ins * 1024
+    shares = [float(x) for x in str(args.insert_share).split(",")]
+    n_ins_all = [int(round(n_req * sh)) if sh > 0 else 0 for sh in shares]
+    n_ins_all = [max(1, x) if sh > 0 else 0 for x, sh in zip(n_ins_all, shares)]
+    extra = sum(n_ins_all) * 1024
     base, queries, params = datasets.workload(args.workload, n=n + extra, nq=args.nq or 20000)
     dim = base.shape[1]
     mcode = abi.L2 if params["metric"] == "l2" else abi.IP
@@ -282,7 +286,11 @@ def run_serving(args):
     idx.set_base(base[:n])
     idx.init_build(levels)
     tb = time.perf_counter()
-    idx.insert(np.arange(n, dtype=np.int32), max_batch=args.insert_batch)
+    chunk = 1 << 20
+    for s0 in range(0, n, chunk):
+        idx.insert(np.arange(s0, min(n, s0 + chunk), dtype=np.int32), max_batch=args.insert_batch)
+        if n > chunk:
+            log(f"  inserted {min(n, s0 + chunk):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
     build_s = time.perf_counter() - tb
     log(f"serving index: {n} x {dim} {base.dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved)")
 
@@ -299,82 +307,92 @@ def run_serving(args):
     log(f"capacity ~{cap_qps:,.0f} QPS at ef=136; offered load {args.load:.2f} -> {rate:,.1f} requests/s")
 
     rng = np.random.default_rng(args.seed)
-    kinds = rng.permutation(np.r_[np.ones(n_ins, bool), np.zeros(n_req - n_ins, bool)])
-    gaps = rng.exponential(1.0 / rate, n_req)
-    sizes = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4096), n_req)))).astype(int)
-    efs = rng.integers(16, 257, n_req)
-    ks = rng.choice([10, 100], n_req)
-    efs = np.maximum(efs, ks)
-    srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=256)
-    tickets = []
-    done = []
-    t0 = time.perf_counter()
-
-    def producer():
-        t_next = t0
-        q_at, ins_at = 0, n
-        for i in range(n_req):
-            t_next += gaps[i]
-            delay = t_next - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            if kinds[i]:
-                ids = np.arange(ins_at, ins_at + 1024, dtype=np.int32)
-                tickets.append(("insert", 1024, srv.insert(ids, base[ins_at:ins_at + 1024])))
-                ins_at += 1024
-            else:
-                m = int(sizes[i])
-                s = q_at % (len(queries) - m)
-                tickets.append(("search", m, srv.submit(queries[s:s + m], int(ks[i]), int(efs[i]))))
-                q_at += m
-        tickets.append(None)
-
+    ins_at = n
+    mixes = []
     slog = []
-    th = threading.Thread(target=producer)
-    th.start()
-    j = 0
-    while True:
-        while j >= len(tickets):
-            time.sleep(0.0005)
-        t = tickets[j]
-        j += 1
-        if t is None:
-            break
-        kind, m, tk = t
-        _, _, _, lat, que = srv.wait(tk)
-        done.append((kind, m, lat, que))
-        if args.serve_log:
-            slog.append({"i": j - 1, "kind": kind, "size": m, "latency_us": round(lat, 1), "queue_us": round(que, 1),
-                         "done_s": round(time.perf_counter() - t0, 6)})
-    th.join()
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
+    for mix, (share, n_ins) in enumerate(zip(shares, n_ins_all)):
+        kinds = rng.permutation(np.r_[np.ones(n_ins, bool), np.zeros(n_req - n_ins, bool)])
+        gaps = rng.exponential(1.0 / rate, n_req)
+        sizes = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4096), n_req)))).astype(int)
+        efs = rng.integers(16, 257, n_req)
+        ks = rng.choice([10, 100], n_req)
+        efs = np.maximum(efs, ks)
+        srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=256)
+        tickets = []
+        done = []
+        t0 = time.perf_counter()
+
+        def producer(ins_first):
+            t_next = t0
+            q_at, at = 0, ins_first
+            for i in range(n_req):
+                t_next += gaps[i]
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                if kinds[i]:
+                    ids = np.arange(at, at + 1024, dtype=np.int32)
+                    tickets.append(("insert", 1024, srv.insert(ids, base[at:at + 1024])))
+                    at += 1024
+                else:
+                    m = int(sizes[i])
+                    s0 = q_at % (len(queries) - m)
+                    tickets.append(("search", m, srv.submit(queries[s0:s0 + m], int(ks[i]), int(efs[i]))))
+                    q_at += m
+            tickets.append(None)
+
+        th = threading.Thread(target=producer, args=(ins_at,))
+        th.start()
+        j = 0
+        while True:
+            while j >= len(tickets):
+                time.sleep(0.0005)
+            t = tickets[j]
+            j += 1
+            if t is None:
+                break
+            kind, m, tk = t
+            _, _, _, lat, que = srv.wait(tk)
+            done.append((kind, m, lat, que))
+            if args.serve_log:
+                slog.append({"mix": mix, "i": j - 1, "kind": kind, "size": m, "latency_us": round(lat, 1),
+                             "queue_us": round(que, 1), "done_s": round(time.perf_counter() - t0, 6)})
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        ins_at += n_ins * 1024
+        lat_s = np.array([d[2] for d in done if d[0] == "search"]) / 1e3
+        lat_i = np.array([d[2] for d in done if d[0] == "insert"]) / 1e3
+        nq_done = sum(d[1] for d in done if d[0] == "search")
+        res = {"insert_share": share, "requests": n_req, "inserts": n_ins, "qps": round(nq_done / wall, 1),
+               "p50_request_latency_ms": round(float(np.percentile(lat_s, 50)), 3),
+               "p99_request_latency_ms": round(float(np.percentile(lat_s, 99)), 3),
+               "p50_insert_latency_ms": round(float(np.percentile(lat_i, 50)), 3) if len(lat_i) else None,
+               "queries": int(nq_done), "wall_s": round(wall, 3), "rows_after": int(ins_at), "loop": st}
+        mixes.append(res)
+        log(f"mix {mix} (insert share {share}): served {nq_done} queries in {n_req - n_ins} requests + {n_ins} "
+            f"inserts in {wall:.2f}s: {res['qps']:,.0f} QPS, request latency p50 {res['p50_request_latency_ms']} ms, "
+            f"p99 {res['p99_request_latency_ms']} ms; loop {st}")
     if args.serve_log:
         with open(args.serve_log, "w") as f:
             for rec in slog:
                 f.write(json.dumps(rec) + "\n")
-    lat_s = np.array([d[2] for d in done if d[0] == "search"]) / 1e3
-    lat_i = np.array([d[2] for d in done if d[0] == "insert"]) / 1e3
-    nq_done = sum(d[1] for d in done if d[0] == "search")
+    first = mixes[0]
     line = {
         "metric": "serving: p50 / p99 request latency and completed QPS under open-loop Poisson arrivals",
-        "value": round(nq_done / wall, 1), "unit": "queries/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
-        "ms_per_step": round(wall / n_req * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "value": first["qps"], "unit": "queries/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
+        "ms_per_step": round(first["wall_s"] / n_req * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": str(base.dtype), "data": "synthetic",
         "config": {"workload": f"{wl['desc']} (n={n}, GPU-built, +{extra} streamed)", "requests": n_req,
-                   "inserts": n_ins, "insert_batch": 1024, "slots": args.slots, "offered_load": args.load,
+                   "insert_batch": 1024, "slots": args.slots, "offered_load": args.load,
                    "arrival_rate_per_s": round(rate, 1), "request_sizes": "log-uniform [1, 4096]",
                    "ef": "uniform [16, 256]", "k": "{10, 100}",
-                   "p50_request_latency_ms": round(float(np.percentile(lat_s, 50)), 3),
-                   "p99_request_latency_ms": round(float(np.percentile(lat_s, 99)), 3),
-                   "p50_insert_latency_ms": round(float(np.percentile(lat_i, 50)), 3) if len(lat_i) else None,
-                   "queries": int(nq_done), "wall_s": round(wall, 3), "loop": st, "build_seconds": round(build_s, 2)},
+                   "p50_request_latency_ms": first["p50_request_latency_ms"],
+                   "p99_request_latency_ms": first["p99_request_latency_ms"],
+                   "mixes": mixes, "build_seconds": round(build_s, 2)},
         "roofline": None, "cpu_baseline": None,
     }
-    log(f"served {nq_done} queries in {n_req - n_ins} requests + {n_ins} inserts in {wall:.2f}s: "
-        f"{nq_done / wall:,.0f} QPS, request latency p50 {line['config']['p50_request_latency_ms']} ms, "
-        f"p99 {line['config']['p99_request_latency_ms']} ms; loop {st}")
     print(json.dumps(line), flush=True)
 
 
@@ -535,7 +553,8 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
     ap.add_argument("--requests", type=int, default=2000)
-    ap.add_argument("--insert-share", type=float, default=0.05)
+    ap.add_argument("--insert-share", default="0,0.05",
+                    help="comma list of insert shares, one serving run each on the same index")
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--serve-log", default=None, help="per-request latencies (JSON lines) of a serving run")

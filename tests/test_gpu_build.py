@@ -55,7 +55,7 @@ def recall(ids, gt, k):
 @pytest.mark.parametrize("reference_rows", [True, False])
 def test_batched_build_recall_close_to_cpu_graph(reference_rows):
     """Batched GPU insertion (batches up to 4096, concurrent AddEdge under
-    per-node locks) judged by its recall-vs-ef curve against the graph the
+    per-node locks) measured by its recall-vs-ef curve against the graph the
     reference's algorithm builds one node at a time.  Both builds depend on
     the insertion order and, on the GPU, on batching, so one pair of 20k-node
     graphs differs by about 1% at a single ef either way; the bound is on the

@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "QPS at recall@10>=0.95 (1 GPU vs host CPU); p99 latency; achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 CALIB_BYTES = 256 << 20
+STREAM_ROWS = 20_000_000  # serving builds of at least this many rows generate them while inserting
 PROFILES = os.path.join(ROOT, "profiles")
 
 WORKLOADS = {
@@ -278,21 +279,66 @@ def run_serving(args):
     n_ins_all = [int(round(n_req * sh)) if sh > 0 else 0 for sh in shares]
     n_ins_all = [max(1, x) if sh > 0 else 0 for x, sh in zip(n_ins_all, shares)]
     extra = sum(n_ins_all) * 1024
-    base, queries, params = datasets.workload(args.workload, n=n + extra, nq=args.nq or 20000)
-    dim = base.shape[1]
+    streamed = n + extra >= STREAM_ROWS
+    if streamed:
+        # rows are generated on a second thread while earlier ones are uploaded
+        # and inserted (the generator releases the interpreter lock)
+        st, params = datasets.workload_stream(args.workload, n=n + extra, nq=args.nq or 20000)
+        dim, dtype = st.centers.shape[1], np.dtype(st.args["out_dtype"])
+        base = None
+    else:
+        base, queries, params = datasets.workload(args.workload, n=n + extra, nq=args.nq or 20000)
+        dim, dtype = base.shape[1], base.dtype
     mcode = abi.L2 if params["metric"] == "l2" else abi.IP
-    idx = abi.Index(base.dtype, mcode, dim, params["M"], params["efc"], n + extra)
+    idx = abi.Index(dtype, mcode, dim, params["M"], params["efc"], n + extra)
     levels = abi.sample_levels(n + extra, params["M"], args.seed)
-    idx.set_base(base[:n])
-    idx.init_build(levels)
     tb = time.perf_counter()
-    chunk = 1 << 20
-    for s0 in range(0, n, chunk):
-        idx.insert(np.arange(s0, min(n, s0 + chunk), dtype=np.int32), max_batch=args.insert_batch)
-        if n > chunk:
-            log(f"  inserted {min(n, s0 + chunk):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
+    if streamed:
+        import queue
+
+        chunks = queue.Queue(maxsize=2)
+
+        def generate():
+            for c in st.base_chunks():
+                chunks.put(c)
+            chunks.put(None)
+
+        tg = threading.Thread(target=generate, daemon=True)
+        tg.start()
+        tail, at, started = [], 0, False
+        while True:
+            c = chunks.get()
+            if c is None:
+                break
+            m = c.shape[0]
+            nb = max(0, min(m, n - at))
+            if nb:
+                idx.set_base(c[:nb], first=at)
+                if not started:
+                    idx.init_build(levels)
+                    started = True
+                idx.insert(np.arange(at, at + nb, dtype=np.int32), max_batch=args.insert_batch,
+                           exact_rows=not args.capped_rows)
+            if nb < m:
+                tail.append(c[nb:])
+            at += m
+            log(f"  inserted {min(at, n):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
+        tg.join()
+        queries = st.queries()
+        extra_rows = np.concatenate(tail) if tail else np.zeros((0, dim), dtype)
+    else:
+        idx.set_base(base[:n])
+        idx.init_build(levels)
+        chunk = 1 << 20
+        for s0 in range(0, n, chunk):
+            idx.insert(np.arange(s0, min(n, s0 + chunk), dtype=np.int32), max_batch=args.insert_batch,
+                       exact_rows=not args.capped_rows)
+            if n > chunk:
+                log(f"  inserted {min(n, s0 + chunk):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
+        extra_rows = base[n:]
     build_s = time.perf_counter() - tb
-    log(f"serving index: {n} x {dim} {base.dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved)")
+    log(f"serving index: {n} x {dim} {dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved"
+        f"{', rows capped at D' if args.capped_rows else ''})")
 
     # capacity probe: one slot-sized launch at the mean ef
     d_q = abi.DeviceBuffer.from_array(queries[:4096])
@@ -332,7 +378,7 @@ def run_serving(args):
                     time.sleep(delay)
                 if kinds[i]:
                     ids = np.arange(at, at + 1024, dtype=np.int32)
-                    tickets.append(("insert", 1024, srv.insert(ids, base[at:at + 1024])))
+                    tickets.append(("insert", 1024, srv.insert(ids, extra_rows[at - n:at - n + 1024])))
                     at += 1024
                 else:
                     m = int(sizes[i])
@@ -383,14 +429,14 @@ def run_serving(args):
         "metric": "serving: p50 / p99 request latency and completed QPS under open-loop Poisson arrivals",
         "value": first["qps"], "unit": "queries/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
         "ms_per_step": round(first["wall_s"] / n_req * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": str(base.dtype), "data": "synthetic",
+        "vs_baseline": None, "dtype": str(dtype), "data": "synthetic",
         "config": {"workload": f"{wl['desc']} (n={n}, GPU-built, +{extra} streamed)", "requests": n_req,
                    "insert_batch": 1024, "slots": args.slots, "offered_load": args.load,
                    "arrival_rate_per_s": round(rate, 1), "request_sizes": "log-uniform [1, 4096]",
                    "ef": "uniform [16, 256]", "k": "{10, 100}",
                    "p50_request_latency_ms": first["p50_request_latency_ms"],
                    "p99_request_latency_ms": first["p99_request_latency_ms"],
-                   "mixes": mixes, "build_seconds": round(build_s, 2)},
+                   "mixes": mixes, "build_seconds": round(build_s, 2), "rows_capped_at_D": bool(args.capped_rows)},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -553,6 +599,8 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
     ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--capped-rows", action="store_true",
+                    help="serving build: rows never exceed D ids (the reference's AddEdge can grow a row past D)")
     ap.add_argument("--insert-share", default="0,0.05",
                     help="comma list of insert shares, one serving run each on the same index")
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")

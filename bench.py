@@ -135,17 +135,17 @@ def pmc_profile_path(workload, ef):
 
 
 def profiled_traffic(workload, n, nq, k, ef):
-    """HBM bytes per k_search launch recorded by a --summarize-pmc run of the
-    same workload, size and ef (None if there is none)."""
+    """HBM bytes per k_search launch and its L2 hit rate, recorded by a
+    --summarize-pmc run of the same workload, size and ef (None if none)."""
     path = pmc_profile_path(workload, ef)
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     for line in open(path):
         if line.startswith("traffic-json: "):
             rec = json.loads(line[len("traffic-json: "):])
             if (rec["n"], rec["nq"], rec["k"], rec["ef"]) == (n, nq, k, ef):
-                return rec["traffic_bytes"], os.path.relpath(path, ROOT)
-    return None, None
+                return rec["traffic_bytes"], os.path.relpath(path, ROOT), rec.get("l2_hit_rate")
+    return None, None, None
 
 
 def _csv_rows(directory, suffix):
@@ -253,6 +253,19 @@ def summarize(args):
         rec = {"workload": args.workload, "n": cfg["n"], "nq": cfg["nq"], "k": cfg["k"], "ef": cfg["ef"],
                "fetch_bytes": int(fb), "write_bytes": int(wb), "traffic_bytes": int(fb + wb),
                "algorithmic_bytes": alg, "factor_fetch": round(ff, 5), "factor_write": round(fw, 5)}
+        if args.summarize_l2:
+            hits = _counter_per_dispatch(args.summarize_l2, "TCC_HIT_sum", last=bl["steps"])
+            miss = _counter_per_dispatch(args.summarize_l2, "TCC_MISS_sum", last=bl["steps"])
+            body += ["L2 (TCC) requests, a third `--pmc` pass (TCC_HIT_sum, TCC_MISS_sum per dispatch):", "",
+                     "| Kernel | hits | misses | hit rate |", "|---|---|---|---|"]
+            for key in sorted(set(hits) & set(miss)):
+                hv, mv = sum(hits[key]) / len(hits[key]), sum(miss[key]) / len(miss[key])
+                rate = hv / (hv + mv) if hv + mv else 0.0
+                body.append(f"| `{key}` | {hv:,.0f} | {mv:,.0f} | {rate * 100:.1f} % |")
+            hk = [key for key in hits if key.startswith("k_search")][0]
+            hv, mv = sum(hits[hk]) / len(hits[hk]), sum(miss[hk]) / len(miss[hk])
+            rec["l2_hit_rate"] = round(hv / (hv + mv), 4)
+            body += ["", f"k_search L2 hit rate: {rec['l2_hit_rate'] * 100:.1f} % of its L2 requests.", ""]
         body.append("traffic-json: " + json.dumps(rec))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
@@ -608,6 +621,8 @@ def main():
     ap.add_argument("--serve-log", default=None, help="per-request latencies (JSON lines) of a serving run")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",
                     help="write a profiles/ summary from a rocprofv3 --kernel-trace --stats output directory")
+    ap.add_argument("--summarize-l2", default=None, metavar="TCC_DIR",
+                    help="with --summarize-pmc: a --pmc TCC_HIT_sum TCC_MISS_sum run's directory")
     ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write a profiles/ summary from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE directories")
     ap.add_argument("--bench-json", default=None, help="the JSON line bench.py printed in the profiled run")
@@ -895,6 +910,18 @@ def main():
     grid, lds = h_main.last_launch()
     log(f"timed: {args.steps} x {nq} queries at ef={chosen}: {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
+    # per-result distance error against FP64 sums of the same (query, row)
+    # pairs, on the ground-truth queries (the timed launches' outputs)
+    r_ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
+    r_d = d_dist.to_array(np.float32, (nq, k))[:gt_nq].astype(np.float64)
+    ok = r_ids >= 0
+    rows = base[np.where(ok, r_ids, 0)].astype(np.float64)
+    qv = queries[:gt_nq].astype(np.float64)[:, None, :]
+    d64 = ((rows - qv) ** 2).sum(-1) if metric == "l2" else -(rows * qv).sum(-1)
+    rel = np.abs(r_d - d64) / np.maximum(np.abs(d64), 1e-30)
+    dist_err = {"max_rel": float(rel[ok].max()), "mean_rel": float(rel[ok].mean()), "queries": int(gt_nq)}
+    log(f"returned distances vs FP64: max relative error {dist_err['max_rel']:.3g}, "
+        f"mean {dist_err['mean_rel']:.3g}")
     if comparison:
         comparison["speedup_vs_fp32_traversal"] = round(qps / comparison["fp32_traversal"]["qps"], 3)
         log(f"{prec} traversal + FP32 re-rank: {qps:,.0f} QPS at recall {recall:.4f} vs FP32 traversal "
@@ -944,7 +971,8 @@ def main():
             log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads at ef={cpu_ef}, recall {cpu_recall:.4f}")
         cpu = entry
 
-    traffic, traffic_src = profiled_traffic(args.workload, n, nq, k, chosen) if prec == "none" else (None, None)
+    traffic, traffic_src, l2_hit = (profiled_traffic(args.workload, n, nq, k, chosen) if prec == "none"
+                                    else (None, None, None))
     search_desc = "exact (reference heap order)" if prec == "none" else \
         f"{prec} traversal copy + exact FP32 re-rank of the ef candidates"
     dtype_out = dtype_name if prec == "none" else f"{prec} traversal, f32 re-rank"
@@ -954,6 +982,7 @@ def main():
            f"GPU batched build ({build_info['build_seconds']:.1f}s)", "search": search_desc,
            "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
            "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
+           "distance_error_vs_fp64": dist_err,
            "batch_latency_ms": round(wall / args.steps * 1e3, 4), "sweep_ef_recall_qps": sweep_out}
     if comparison:
         cfg["precision_comparison"] = comparison
@@ -965,7 +994,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": dtype_out, "data": "synthetic", "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "traffic_source": traffic_src,
+                     "traffic_source": traffic_src, "l2_hit_rate": l2_hit,
                      "kernel": "k_search" if prec == "none" else "k_quantize + k_search + k_rerank",
                      "kernel_ms": round(kernel_s * 1e3, 4), "algorithmic_bytes_per_launch": alg_bytes,
                      "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),

@@ -36,8 +36,8 @@ def test_profiled_traffic_lookup(tmp_path, monkeypatch):
     rec = {"workload": "w", "n": 10, "nq": 2, "k": 10, "ef": 96, "traffic_bytes": 1234}
     (tmp_path / "k_search_w_ef96_pmc.md").write_text("text\ntraffic-json: " + json.dumps(rec) + "\n")
     assert bench.profiled_traffic("w", 10, 2, 10, 96)[0] == 1234
-    assert bench.profiled_traffic("w", 11, 2, 10, 96) == (None, None)  # another size
-    assert bench.profiled_traffic("w", 10, 2, 10, 64) == (None, None)  # another ef
+    assert bench.profiled_traffic("w", 11, 2, 10, 96) == (None, None, None)  # another size
+    assert bench.profiled_traffic("w", 10, 2, 10, 64) == (None, None, None)  # another ef
 
 
 def test_more_than_one_gpu_is_refused():

@@ -104,6 +104,9 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++) coff[kb] = kb * 32 + h * 16 < vstride ? kb * 32 + h * 16 : 0;
 
+    __shared__ int skey_all[4][16 * kWave];
+    int* skey = skey_all[threadIdx.x >> 6];
+
     int kd[KB], ki[KB];
 #pragma unroll
     for (int j = 0; j < KB; j++) {
@@ -146,28 +149,48 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
 #pragma unroll
         for (int kb = 0; kb < NKB; kb++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kb], bq[kb], acc, 0, 0, 0);
 
+        // Lanes l and l + 32 hold the same query.  A key above the other
+        // half's 32nd-best key loses to all 32 of that half's entries, so it
+        // cannot be among the query's 32 best: the filter takes the tighter
+        // bound (equal keys pass; the merge orders ties by row id).
+        const int other = __shfl_xor(kd[KB - 1], 32, kWave);
+        const int obound = other == INT_MAX ? INT_MAX : other + 1;
+        thr = min(kd[KB - 1], obound);
+
         // acc[r] is row t*32 + 8 (r / 4) + 4 h + r % 4 against query col
         int key[16];
-        bool any = false;
+        uint32_t pend = 0;  // rows of this tile that pass the lane's filter
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int4 c4 = cr[r >> 2];
             const int c = (r & 3) == 0 ? c4.x : (r & 3) == 1 ? c4.y : (r & 3) == 2 ? c4.z : c4.w;
             key[r] = (CROW ? c : 0) - COEF * acc[r];
-            any |= key[r] < thr;
+            pend |= (uint32_t)(key[r] < thr) << r;
         }
-        if (ballot(any)) {
-            // one copy of the insert: take key[0], then shift the rest down
-#pragma unroll 1
-            for (int r = 0; r < 16; r++) {
-                const int64_t m = t * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (key[0] < thr && m < nbase) {
-                    sorted_insert(kd, ki, key[0], (int)m);
-                    thr = kd[KB - 1];
-                }
+        // every lane inserts its next passing row in the same step, in row
+        // order, re-checking against its tightened bound: a step per row
+        // of the busiest lane, not per row that any lane passes.  The keys
+        // wait in LDS ([row][lane], no bank conflicts) so that registers
+        // need not hold all sixteen through the loop.
+        if (ballot(pend != 0)) {
 #pragma unroll
-                for (int i = 0; i < 15; i++) key[i] = key[i + 1];
+            for (int r = 0; r < 16; r++) skey[r * kWave + lane] = key[r];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+            for (int step = 0; step < 16; step++) {
+                if (!ballot(pend != 0)) break;
+                if (pend) {
+                    const int r = __builtin_ctz(pend);
+                    pend &= pend - 1;
+                    const int kr = skey[r * kWave + lane];
+                    const int64_t m = t * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                    if (kr < thr && m < nbase) {
+                        sorted_insert(kd, ki, kr, (int)m);
+                        thr = min(kd[KB - 1], obound);
+                    }
+                }
             }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 

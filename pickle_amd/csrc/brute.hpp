@@ -23,23 +23,12 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Branch-free selects: nested ?: chains over unrolled register arrays come
-// out of the front end as branches, which stay branches after optimisation.
-__device__ __forceinline__ int sel_bits(bool c, int a, int b) {
-    const int m = -(int)c;
-    return (a & m) | (b & ~m);
-}
-__device__ __forceinline__ double sel_bits(bool c, double a, double b) {
-    const long long m = -(long long)c;
-    return __longlong_as_double((__double_as_longlong(a) & m) | (__double_as_longlong(b) & ~m));
-}
-
 // Sorted ascending insert of (d, id), dropping the last entry.  The caller
 // only inserts ids larger than every id in the list, so the new entry goes
-// after any equal keys and the list stays sorted by (key, id).  Written as
-// a shift with no dependence between positions: entry j keeps its value
-// while it is <= d, the first larger position takes d, and the rest take
-// their predecessor.
+// after any equal keys and the list stays sorted by (key, id).  Keys shift
+// by min / max (entry j becomes max(kd[j-1], min(kd[j], d))); ids follow
+// with one select each, from the same comparisons, no position depending
+// on another's result.
 template <typename K, int KB>
 __device__ __forceinline__ void sorted_insert(K (&kd)[KB], int (&ki)[KB], K d, int id) {
     bool le[KB];
@@ -47,11 +36,12 @@ __device__ __forceinline__ void sorted_insert(K (&kd)[KB], int (&ki)[KB], K d, i
     for (int j = 0; j < KB; j++) le[j] = kd[j] <= d;
 #pragma unroll
     for (int j = KB - 1; j > 0; j--) {
-        kd[j] = sel_bits(le[j], kd[j], sel_bits(le[j - 1], d, kd[j - 1]));
-        ki[j] = sel_bits(le[j], ki[j], sel_bits(le[j - 1], id, ki[j - 1]));
+        const int from = le[j - 1] ? id : ki[j - 1];
+        ki[j] = le[j] ? ki[j] : from;
+        kd[j] = max(kd[j - 1], min(kd[j], d));
     }
-    kd[0] = sel_bits(le[0], kd[0], d);
-    ki[0] = sel_bits(le[0], ki[0], id);
+    ki[0] = le[0] ? ki[0] : id;
+    kd[0] = min(kd[0], d);
 }
 
 __device__ __forceinline__ v4i as_v4i(uint4 u, uint32_t flip) {

@@ -38,6 +38,11 @@
 
 namespace pk {
 
+__global__ void k_brute_fill(int32_t* __restrict__ p, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---- per-row constants -------------------------------------------------
 
 // integer rows: |x'|^2 (L2) or -128 sum x' (u8 -IP); rows past nbase get 0
@@ -75,7 +80,8 @@ template <int DT, int MET, int NKB>
 __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
                                                     const int32_t* __restrict__ crow, const uint8_t* __restrict__ qs,
                                                     int64_t nq, int dim, int64_t tiles_per_split, int nlists,
-                                                    double* __restrict__ pd, int32_t* __restrict__ pi) {
+                                                    double* __restrict__ pd, int32_t* __restrict__ pi,
+                                                    int32_t* __restrict__ qbound) {
     constexpr int KB = kBruteK;
     constexpr int QST = NKB * 32;
     constexpr uint32_t FLIP = DT == DT_U8 ? 0x80808080u : 0u;
@@ -114,46 +120,50 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
         ki[j] = INT_MAX;
     }
     int thr = INT_MAX;
+    int gbound = INT_MAX;  // smallest 32nd-best key any list of this query has published
 
     const int64_t ntiles = (nbase + 31) / 32;
     const int64_t t0 = (int64_t)blockIdx.y * tiles_per_split;
     const int64_t t1 = t0 + tiles_per_split < ntiles ? t0 + tiles_per_split : ntiles;
 
-    v4i an[NKB];
-    int4 cn[4];
-    auto load_tile = [&](int64_t t) {
+    // Base-row tiles arrive two ahead of their use (three register
+    // buffers in rotation): one tile of MFMA and selection work is shorter
+    // than a gather's latency at two waves per SIMD.
+    struct Tile {
+        v4i a[NKB];
+        int4 c[4];
+    };
+    auto load_tile = [&](Tile& T, int64_t t) {
         int64_t row = t * 32 + col;
         row = row < nbase ? row : nbase - 1;
         const uint8_t* rp = base + row * (int64_t)vstride;
 #pragma unroll
-        for (int kb = 0; kb < NKB; kb++) an[kb] = as_v4i(ld16_global(rp + coff[kb]), FLIP);
+        for (int kb = 0; kb < NKB; kb++) T.a[kb] = as_v4i(ld16_global(rp + coff[kb]), FLIP);
         if (CROW) {
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const uint4 u = ld16_global(crow + t * 32 + 8 * g + 4 * h);
-                cn[g] = make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
+                T.c[g] = make_int4((int)u.x, (int)u.y, (int)u.z, (int)u.w);
             }
         }
     };
-    if (t0 < t1) load_tile(t0);
-    for (int64_t t = t0; t < t1; t++) {
-        v4i a[NKB];
-#pragma unroll
-        for (int kb = 0; kb < NKB; kb++) a[kb] = an[kb];
-        int4 cr[4];
-#pragma unroll
-        for (int g = 0; g < 4; g++) cr[g] = cn[g];
-        if (t + 1 < t1) load_tile(t + 1);
-
+    auto process = [&](const Tile& T, int64_t t) {
         v16i acc = {};
 #pragma unroll
-        for (int kb = 0; kb < NKB; kb++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kb], bq[kb], acc, 0, 0, 0);
+        for (int kb = 0; kb < NKB; kb++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.a[kb], bq[kb], acc, 0, 0, 0);
 
         // Lanes l and l + 32 hold the same query.  A key above the other
         // half's 32nd-best key loses to all 32 of that half's entries, so it
         // cannot be among the query's 32 best: the filter takes the tighter
         // bound (equal keys pass; the merge orders ties by row id).
-        const int other = __shfl_xor(kd[KB - 1], 32, kWave);
+        // Every 32 tiles the lane also publishes its 32nd-best key to the
+        // query's bound in device memory (shared by all row splits) and
+        // takes back the smallest published so far, for the same reason.
+        if (((t - t0) & 31) == 31 && q < nq) {
+            const int got = atomicMin(&qbound[q], kd[KB - 1]);
+            gbound = min(gbound, min(got, kd[KB - 1]));
+        }
+        const int other = min(__shfl_xor(kd[KB - 1], 32, kWave), gbound);
         const int obound = other == INT_MAX ? INT_MAX : other + 1;
         thr = min(kd[KB - 1], obound);
 
@@ -162,7 +172,7 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
         uint32_t pend = 0;  // rows of this tile that pass the lane's filter
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const int4 c4 = cr[r >> 2];
+            const int4 c4 = T.c[r >> 2];
             const int c = (r & 3) == 0 ? c4.x : (r & 3) == 1 ? c4.y : (r & 3) == 2 ? c4.z : c4.w;
             key[r] = (CROW ? c : 0) - COEF * acc[r];
             pend |= (uint32_t)(key[r] < thr) << r;
@@ -192,6 +202,19 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
             }
             __builtin_amdgcn_wave_barrier();
         }
+    };
+    Tile B0, B1, B2;
+    if (t0 < t1) load_tile(B0, t0);
+    if (t0 + 1 < t1) load_tile(B1, t0 + 1);
+    for (int64_t t = t0; t < t1; t += 3) {
+        if (t + 2 < t1) load_tile(B2, t + 2);
+        process(B0, t);
+        if (t + 1 >= t1) break;
+        if (t + 3 < t1) load_tile(B0, t + 3);
+        process(B1, t + 1);
+        if (t + 2 >= t1) break;
+        if (t + 4 < t1) load_tile(B1, t + 4);
+        process(B2, t + 2);
     }
 
     if (q < nq) {
@@ -249,12 +272,12 @@ __global__ __launch_bounds__(256) void k_brute_merge(const double* __restrict__ 
 template <int DT, int MET>
 static hipError_t launch_int(int nkb, const BrutePlan& p, hipStream_t s, const uint8_t* base, int64_t nbase,
                              int vstride, const int32_t* crow, const uint8_t* dq, int64_t nq, int dim, double* pd,
-                             int32_t* pi) {
+                             int32_t* pi, int32_t* qbound) {
     dim3 grid((unsigned)p.qblocks, (unsigned)p.splits);
 #define PK_IMMA_CASE(N)                                                                                       \
     if (nkb <= N) {                                                                                           \
         hipLaunchKernelGGL((k_brute_imma<DT, MET, N>), grid, dim3(256), 0, s, base, nbase, vstride, crow, dq, \
-                           nq, dim, p.tiles_per_split, p.nlists, pd, pi);                                    \
+                           nq, dim, p.tiles_per_split, p.nlists, pd, pi, qbound);                            \
         return hipGetLastError();                                                                             \
     }
     PK_IMMA_CASE(1)
@@ -298,19 +321,22 @@ extern "C" int pk_brute_force(pk_index* h, const void* queries, int64_t nq, int3
     int32_t* pi = nullptr;
     int32_t* di = nullptr;
     float* dd = nullptr;
+    int32_t* qb = nullptr;  // per-query bound shared by the integer tiles' row splits
     auto cleanup = [&]() {
-        for (void* x : {(void*)dq, rc, (void*)pd, (void*)pi, (void*)di, (void*)dd})
+        for (void* x : {(void*)dq, rc, (void*)pd, (void*)pi, (void*)di, (void*)dd, (void*)qb})
             if (x) (void)hipFree(x);
     };
     const size_t nl = (size_t)nq * p.nlists * kBruteK;
     if (hipMalloc(&dq, staged.size()) != hipSuccess || hipMalloc(&rc, (size_t)npad * 8) != hipSuccess ||
         hipMalloc(&pd, nl * 8) != hipSuccess || hipMalloc(&pi, nl * 4) != hipSuccess ||
-        hipMalloc(&di, (size_t)nq * kBruteK * 4) != hipSuccess || hipMalloc(&dd, (size_t)nq * kBruteK * 4) != hipSuccess) {
+        hipMalloc(&di, (size_t)nq * kBruteK * 4) != hipSuccess || hipMalloc(&dd, (size_t)nq * kBruteK * 4) != hipSuccess ||
+        hipMalloc(&qb, (size_t)nq * 4) != hipSuccess) {
         cleanup();
         return set_error(PK_ERR_NOMEM, "brute force: device allocation failed");
     }
     hipStream_t s = h->stream;
     hipError_t e = hipMemcpyAsync(dq, staged.data(), staged.size(), hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(k_brute_fill, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, qb, INT_MAX, nq);
     const dim3 rgrid((unsigned)(npad / 256 + 1));
     const uint8_t* b = h->base;
     const int vs = h->vstride;
@@ -318,18 +344,18 @@ extern "C" int pk_brute_force(pk_index* h, const void* queries, int64_t nq, int3
         switch (h->dtype * 4 + h->metric) {
             case DT_U8 * 4 + MET_L2:
                 hipLaunchKernelGGL((k_brute_rowconst<DT_U8, MET_L2>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
-                e = launch_int<DT_U8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                e = launch_int<DT_U8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi, qb);
                 break;
             case DT_U8 * 4 + MET_IP:
                 hipLaunchKernelGGL((k_brute_rowconst<DT_U8, MET_IP>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
-                e = launch_int<DT_U8, MET_IP>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                e = launch_int<DT_U8, MET_IP>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi, qb);
                 break;
             case DT_I8 * 4 + MET_L2:
                 hipLaunchKernelGGL((k_brute_rowconst<DT_I8, MET_L2>), rgrid, dim3(256), 0, s, b, nbase, vs, dim, npad, (int32_t*)rc);
-                e = launch_int<DT_I8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi);
+                e = launch_int<DT_I8, MET_L2>(nu, p, s, b, nbase, vs, (int32_t*)rc, dq, nq, dim, pd, pi, qb);
                 break;
             case DT_I8 * 4 + MET_IP:
-                e = launch_int<DT_I8, MET_IP>(nu, p, s, b, nbase, vs, nullptr, dq, nq, dim, pd, pi);
+                e = launch_int<DT_I8, MET_IP>(nu, p, s, b, nbase, vs, nullptr, dq, nq, dim, pd, pi, qb);
                 break;
             case DT_F32 * 4 + MET_L2:
             case DT_F32 * 4 + MET_IP:

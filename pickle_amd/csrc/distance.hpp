@@ -192,4 +192,60 @@ struct FloatDist {
     }
 };
 
+// Order-free FP16 distance, for the traversal copy of the precision form
+// only (an FP16 index with its FP32 rows attached, whose results the FP32
+// re-rank recomputes in the reference's order).  Groups of 16 lanes; lane g
+// owns 16-byte chunks g, g + 16, ...: one load per lane per row where the
+// reference-order FloatDist needs eight 2-byte loads, so a batch keeps
+// 32 rows in flight.  The eight halves of a chunk widen to FP32 and
+// accumulate per lane; the 16 lanes then sum by the halving tree.  Rows and
+// the staged query are zero past the vector.
+template <int MET, int NCH>
+struct HalfDist {
+    static constexpr int LPC = 16;
+    struct Row {
+        uint4 c[NCH];
+    };
+    float q[NCH * 8];
+    int nchunks;
+
+    __device__ static __forceinline__ float half_at(uint32_t w, int hi) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> (16 * hi)));
+    }
+
+    __device__ void load_query(const uint8_t* qst, int g, int nchunks_) {
+        nchunks = nchunks_;
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            const int c = g + 16 * i;
+            const uint4 u = c < nchunks ? ld16_lds(qst + 16 * c) : make_uint4(0, 0, 0, 0);
+            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) q[8 * i + e] = half_at(w[e >> 1], e & 1);
+        }
+    }
+
+    __device__ void load_row(Row& r, const uint8_t* row, int g, bool /*active*/) const {
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            const int c = g + 16 * i;
+            r.c[i] = ld16_global(row + 16 * (c < nchunks ? c : 0));
+        }
+    }
+
+    // every lane of the group returns the distance
+    __device__ float finish(const Row& r, int g, const uint8_t* /*row*/, bool /*active*/) const {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            if (g + 16 * i >= nchunks) continue;
+            const uint32_t w[4] = {r.c[i].x, r.c[i].y, r.c[i].z, r.c[i].w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) acc = fterm<MET>(q[8 * i + e], half_at(w[e >> 1], e & 1), acc);
+        }
+        const float s = tree_sum16(acc);
+        return MET == MET_IP ? -s : s;
+    }
+};
+
 }  // namespace pk

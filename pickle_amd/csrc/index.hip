@@ -504,6 +504,15 @@ static int default_vbits(int ef_max, int64_t capacity) {
     return std::min(v, cap_bits);
 }
 
+// An FP16 index with FP32 rows attached is a traversal copy: its results
+// are re-ranked in exact FP32, so it searches with the order-free FP16
+// distance (16-byte gathers) instead of the reference's summation order.
+static int search_class(const pk_index* h) {
+    if (h->dtype == PK_FLOAT16 && h->full && h->vstride <= 512)
+        return kHalfFastClass + (h->vstride <= 256 ? 0 : 1);
+    return h->size_class;
+}
+
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
@@ -580,7 +589,7 @@ static int enqueue_search(pk_index* h, SearchCtx& c, const SearchPlan& P) {
         h->last_grid = P.grid;
         h->last_lds = P.lds;
     }
-    HIPCHK(launch_search(P.a, h->dtype, h->metric, h->size_class, P.grid, P.lds, c.stream));
+    HIPCHK(launch_search(P.a, h->dtype, h->metric, search_class(h), P.grid, P.lds, c.stream));
     return PK_OK;
 }
 

@@ -140,6 +140,11 @@ static hipError_t launch_dt(const SearchArgs& a, int size_class, int grid, size_
         if (size_class == 1) return launch_rs<IntDist<DT, MET, 2>, true>(a, grid, lds, s);
         return launch_rs<IntDist<DT, MET, 4>, true>(a, grid, lds, s);
     } else {
+        // classes 8 and 9: the order-free FP16 traversal distance
+        if constexpr (DT == DT_F16) {
+            if (size_class == kHalfFastClass) return launch_rs<HalfDist<MET, 1>, true>(a, grid, lds, s);
+            if (size_class == kHalfFastClass + 1) return launch_rs<HalfDist<MET, 2>, true>(a, grid, lds, s);
+        }
         if (size_class == 0) return launch_rs<FloatDist<DT, MET, 8>, false>(a, grid, lds, s);
         return launch_rs<FloatDist<DT, MET, 16>, false>(a, grid, lds, s);
     }

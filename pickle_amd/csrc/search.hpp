@@ -51,6 +51,9 @@ struct SearchArgs {
 };
 
 int size_class_for(int dtype, int dim, int vstride);
+// size classes of the order-free FP16 traversal distance (rows of at most
+// 256 and 512 bytes), used for FP16 indexes that carry FP32 rows
+constexpr int kHalfFastClass = 8;
 int result_capacity_for(int ef_max);
 hipError_t launch_search(const SearchArgs& a, int dtype, int metric, int size_class, int grid, size_t lds,
                          hipStream_t s);

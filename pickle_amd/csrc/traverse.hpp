@@ -32,6 +32,14 @@ template <int DT, int MET>
 struct PassesOf<FloatDist<DT, MET, 16>> {
     static constexpr int value = 2;
 };
+template <int MET>
+struct PassesOf<HalfDist<MET, 1>> {
+    static constexpr int value = 8;
+};
+template <int MET>
+struct PassesOf<HalfDist<MET, 2>> {
+    static constexpr int value = 4;
+};
 
 // distances of the `ncand` rows whose ids are in cl[]; results to cd[]
 template <class DF>

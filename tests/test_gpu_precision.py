@@ -6,7 +6,14 @@ stored again with the traversal copy's vectors (FP16 by round-to-nearest-even,
 or the int8 quantizer restated in numpy), and the oracle's AnnSearch on it
 with k = ef gives the candidates.  The expected answer is those candidates
 ranked by the oracle's FP32 distance (the reference's summation order),
-ties by candidate rank.  Ids and distances must match bit for bit.
+ties by candidate rank.
+
+int8 traversal distances are exact integers, so ids and distances must match
+bit for bit.  An FP16 copy traverses with its own FP32 summation order (16-byte
+gathers), so its traversal distances agree with the reference order within
+1e-5 relative and an id may differ where two candidates' distances are that
+close: at least 99% of ids must match, and every returned distance must be
+the exact FP32 distance (reference order) of the id returned.
 """
 import sys
 
@@ -86,14 +93,23 @@ def test_rerank_matches_oracle(kind, metric):
     src = index_from_oracle(g32, slack=False)  # the re-layered oracle graphs hold no slack row
     idx = precision.traversal_index(src, base, kind)
     ids, ds, cnt = idx.search_rerank(queries, k, ef)
-    np.testing.assert_array_equal(ids, want_ids)
-    np.testing.assert_array_equal(ds, want_d)
     assert (cnt == k).all()
-    # the traversal copy alone, through the ordinary search, equals the oracle's
-    t_ids, t_d, _ = idx.search(q_c, 10, ef)
+    t_ids, t_d, _ = idx.search(q_c, 10, ef)  # the traversal copy alone, through the ordinary search
     o_ids, o_d, _ = gconv.search(q_c, 10, ef)
-    np.testing.assert_array_equal(t_ids, o_ids)
-    np.testing.assert_array_equal(t_d, o_d)
+    if kind == "i8":
+        np.testing.assert_array_equal(ids, want_ids)
+        np.testing.assert_array_equal(ds, want_d)
+        np.testing.assert_array_equal(t_ids, o_ids)
+        np.testing.assert_array_equal(t_d, o_d)
+    else:
+        assert (ids == want_ids).mean() >= 0.99, (ids == want_ids).mean()
+        exact = np.array([[O.distance(queries[i], base[j], metric) for j in row] for i, row in enumerate(ids)],
+                         np.float32)
+        np.testing.assert_array_equal(ds, exact)  # the re-rank's FP32 values, reference order
+        assert (t_ids == o_ids).mean() >= 0.99, (t_ids == o_ids).mean()
+        ref_t = np.array([[O.distance(q_c[i], base_c[j], metric) for j in row] for i, row in enumerate(t_ids)],
+                         np.float32)
+        np.testing.assert_allclose(t_d, ref_t, rtol=1e-5, atol=1e-6)
     idx.close()
     src.close()
 

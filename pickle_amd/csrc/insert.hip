@@ -721,9 +721,7 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
     while (t < n) {
         // batch: grows with the graph; a node above the current top level goes alone
         const int64_t size0 = h->layers[0].n;
-        // a batch is at most 1/32 of the graph it searches: the nodes of one
-        // batch do not see each other, and larger shares cost recall
-        int64_t allow = std::max<int64_t>(1, std::min<int64_t>(max_batch, size0 / 32));
+        int64_t allow = std::max<int64_t>(1, std::min<int64_t>(max_batch, size0 / 8));
         int64_t s = t;
         int batch_top = 0;
         while (s < n && s - t < allow) {

@@ -55,7 +55,7 @@ def recall(ids, gt, k):
 @pytest.mark.parametrize("reference_rows", [True, False])
 def test_batched_build_recall_close_to_cpu_graph(reference_rows):
     """Batched GPU insertion (batches up to 4096, concurrent AddEdge under
-    per-node locks) measured by its recall-vs-ef curve against the graph the
+    per-node locks; two builds, mean recall) measured by its recall-vs-ef curve against the graph the
     reference's algorithm builds one node at a time.  Both builds depend on
     the insertion order and, on the GPU, on batching, so one pair of 20k-node
     graphs differs by about 1% at a single ef either way; the bound is on the
@@ -72,12 +72,17 @@ def test_batched_build_recall_close_to_cpu_graph(reference_rows):
     g = O.Graph(base, O.L2, M, efc, levels)
     g.build(np.arange(n, dtype=np.int32))
     gt, _ = O.brute_force(base, queries, 10, O.L2)
-    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
-    idx.build(base, levels=levels, max_batch=4096, exact_rows=reference_rows)
-    if not reference_rows:
-        for l in range(idx.info()["nlayers"]):
-            L = idx.read_layer(l)
-            assert (L["deg"] <= L["adj"].shape[1]).all(), "rows stay within D"
+    # two batched builds: the order of concurrent AddEdge calls differs from
+    # build to build, and their mean recall halves that variance
+    builds = []
+    for _ in range(2):
+        idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
+        idx.build(base, levels=levels, max_batch=4096, exact_rows=reference_rows)
+        if not reference_rows:
+            for l in range(idx.info()["nlayers"]):
+                L = idx.read_layer(l)
+                assert (L["deg"] <= L["adj"].shape[1]).all(), "rows stay within D"
+        builds.append(idx)
     cpu_idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
     cpu_idx.set_base(base)
     for l in range(g.nlayers):
@@ -86,12 +91,13 @@ def test_batched_build_recall_close_to_cpu_graph(reference_rows):
     cpu_idx.set_enter_level(g.enter_level)
     gaps = []
     for ef in (10, 20, 40, 80):
-        r_gpu = recall(idx.search(queries, 10, ef)[0], gt, 10)
+        r_gpu = np.mean([recall(b.search(queries, 10, ef)[0], gt, 10) for b in builds])
         r_cpu = recall(cpu_idx.search(queries, 10, ef)[0], gt, 10)
         say(f"  ef={ef} recall gpu-built {r_gpu:.4f} cpu-built {r_cpu:.4f}")
         assert r_gpu >= r_cpu - (0.02 if reference_rows else 0.04), (ef, r_gpu, r_cpu)
         gaps.append(r_gpu - r_cpu)
     say(f"  mean gap {np.mean(gaps):+.4f}")
     assert np.mean(gaps) >= -(0.005 if reference_rows else 0.03), gaps
-    idx.close()
+    for b in builds:
+        b.close()
     cpu_idx.close()

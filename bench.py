@@ -785,6 +785,15 @@ def main():
 
     vbits = {}
 
+    def default_vbits(ef):
+        """The library's visited-table size for an ef (index.hip,
+        default_vbits): by ef, at most twice the capacity."""
+        v = 14 if ef <= 32 else 15 if ef <= 64 else 16 if ef <= 256 else 17 if ef <= 512 else 18
+        cap_bits = 10
+        while cap_bits < 20 and (1 << cap_bits) < 2 * n:
+            cap_bits += 1
+        return min(v, cap_bits)
+
     def sweep(which, target, efs):
         h, run = backend(which)
         out, chosen = [], None
@@ -798,9 +807,9 @@ def main():
                 run(ef)
                 ms = h.timer_end()
                 flags = h.device_error()
-                if flags == 1 and vbits.get(which, 16) < 20:
+                if flags == 1 and vbits.get(which, default_vbits(ef)) < 20:
                     # a query filled its visited table: a larger one per wave
-                    vbits[which] = vbits.get(which, 16) + 1
+                    vbits[which] = max(vbits.get(which, 0), default_vbits(ef)) + 1
                     h.set_tuning(0, vbits[which], 0)
                     log(f"[{which}] ef={ef}: visited table full; retrying with 2^{vbits[which]} slots per wave")
                     continue

@@ -44,3 +44,38 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def _counter_csv(path, rows):
+    path.mkdir(parents=True, exist_ok=True)
+    lines = ["Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value"]
+    lines += [f'{d},"{k}",{c},{v}' for d, k, c, v in rows]  # quoted, as rocprofv3 writes names with commas
+    (path / "run_counter_collection.csv").write_text("\n".join(lines) + "\n")
+
+
+def test_pmc_summary_calibrates_traffic_and_reads_l2_hits(tmp_path):
+    """--summarize-pmc: bytes scaled by what the counters read for the 256 MiB
+    copy, only the timed dispatches of k_search, and the L2 hit rate of the
+    TCC pass, recorded in the traffic line the bench reads back."""
+    kib = bench.CALIB_BYTES / 1024
+    search = "void pk::k_search<pk::IntDist<0, 1, 1>, true>(pk::SearchArgs)"
+    copy = "pk::k_copy16(void*)"
+    # FETCH reads half the copy's bytes; the first k_search dispatch is an untimed one
+    _counter_csv(tmp_path / "f", [(1, search, "FETCH_SIZE", 999999), (2, search, "FETCH_SIZE", 1000),
+                                  (3, copy, "FETCH_SIZE", kib / 2)])
+    _counter_csv(tmp_path / "w", [(1, search, "WRITE_SIZE", 999999), (2, search, "WRITE_SIZE", 300),
+                                  (3, copy, "WRITE_SIZE", kib)])
+    _counter_csv(tmp_path / "l", [(1, search, "TCC_HIT_sum", 1), (1, search, "TCC_MISS_sum", 1),
+                                  (2, search, "TCC_HIT_sum", 25), (2, search, "TCC_MISS_sum", 75)])
+    bl = {"steps": 1, "config": {"workload": "w", "ef": 96, "k": 10, "nq": 2, "n": 10, "recall@10": 0.95},
+          "roofline": {"algorithmic_bytes_per_launch": 1024000}}
+    (tmp_path / "b.json").write_text(json.dumps(bl) + "\n")
+    out = tmp_path / "p.md"
+    args = type("A", (), dict(bench_json=str(tmp_path / "b.json"), cmd=["cmd"], summarize_trace=None,
+                              summarize_pmc=(str(tmp_path / "f"), str(tmp_path / "w")),
+                              summarize_l2=str(tmp_path / "l"), workload="w", out=str(out)))
+    bench.summarize(args)
+    rec = json.loads([x for x in out.read_text().splitlines() if x.startswith("traffic-json: ")][0][14:])
+    assert rec["fetch_bytes"] == 1000 * 1024 * 2 and rec["write_bytes"] == 300 * 1024
+    assert rec["traffic_bytes"] == rec["fetch_bytes"] + rec["write_bytes"]
+    assert rec["l2_hit_rate"] == 0.25

@@ -163,6 +163,7 @@ def _csv_rows(directory, suffix):
 
 def _short_kernel(name):
     for key in ("k_search", "k_brute_imma", "k_brute_dmma", "k_brute_rowconst", "k_brute_rownorm", "k_brute_merge",
+                "k_brute_fill",
                 "k_copy16", "k_gather_rows", "k_fill_i32", "k_insert_search", "k_insert_rows", "k_insert_edges",
                 "k_rerank"):
         if key in name:

@@ -188,14 +188,17 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
 /* Queues nq queries (host rows, copied at once); results go to the caller's
  * out_ids / out_dists (nq x k) and out_count (nq, may be NULL) when the
- * request completes. */
+ * request completes, so those buffers must outlive the request.  A query that
+ * fills its visited table or candidate heap gets count -1 and its request
+ * fails with PK_ERR_DEVICE; the other requests of its launch do not. */
 int pk_server_submit(pk_server *s, const void *queries, int64_t nq, int32_t k, int32_t ef,
                      int32_t *out_ids, float *out_dists, int32_t *out_count, int64_t *ticket);
 /* Queues Insert of n rows (host, by external id; copied at once). */
 int pk_server_insert(pk_server *s, const int32_t *ext_ids, const void *rows, int64_t n, int flags,
                      int64_t *ticket);
 /* Waits for a request (timeout_ms < 0: no limit) and releases its ticket;
- * latency_us = submit to completion, queue_us = submit to admission. */
+ * latency_us = submit to completion, queue_us = submit to admission.  One
+ * caller at a time may wait on a ticket; after a timeout it stays valid. */
 int pk_server_wait(pk_server *s, int64_t ticket, int32_t timeout_ms, double *latency_us, double *queue_us);
 /* out[5] = {launches, queries done, inserts done, most slots busy at once, requests waiting}. */
 int pk_server_stats(pk_server *s, int64_t *out);

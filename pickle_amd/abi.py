@@ -34,6 +34,7 @@ EXPORTS = [
     "pk_search_graph_launch", "pk_search_graph_destroy", "pk_autotune_search", "pk_server_create",
     "pk_server_submit", "pk_server_insert", "pk_server_wait", "pk_server_stats", "pk_server_destroy",
 ]
+ERR_STATE, ERR_DEVICE = -4, -7
 BUILD_EXACT_ROWS = 1
 BRUTE_K = 32
 
@@ -213,7 +214,14 @@ class Server:
     def wait(self, ticket: int, timeout_ms: int = -1):
         """(ids, dists, count, latency_us, queue_us); ids etc. are None for inserts."""
         lat, que = C.c_double(), C.c_double()
-        check(lib().pk_server_wait(self.s, ticket, timeout_ms, C.byref(lat), C.byref(que)))
+        rc = lib().pk_server_wait(self.s, ticket, timeout_ms, C.byref(lat), C.byref(que))
+        if rc != 0:
+            err = PickleError(rc, lib().pk_last_error().decode())
+            if rc != ERR_STATE:  # the request finished (failed) and its ticket is released
+                res = self._keep.pop(ticket, None)
+                if rc == ERR_DEVICE and res is not None:
+                    err.partial = res  # (ids, dists, count): count -1 marks the queries that failed
+            raise err  # PK_ERR_STATE: timeout or another waiter; the ticket stays valid
         res = self._keep.pop(ticket)
         if res is None:
             return None, None, None, lat.value, que.value

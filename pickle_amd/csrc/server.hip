@@ -58,7 +58,9 @@ struct Request {
     int32_t* out_count = nullptr;
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
+    bool waited = false;  // a caller is in pk_server_wait on it
     int status = PK_OK;
+    std::string err;  // the failure's message (set on the thread that saw it)
     // insert request
     bool is_insert = false;
     std::vector<int32_t> ext_ids;
@@ -160,20 +162,41 @@ int launch_slot(pk_server* S, Slot& s) {
     return PK_OK;
 }
 
-// results of a finished slot into the requests' own buffers
+// a launch that failed part-way may still have copies queued that read or
+// write the slot's pinned staging: drain its stream before the slot is reused
+int fail_launch(Slot& s, int rc) {
+    const std::string msg = pk_last_error();
+    (void)hipStreamSynchronize(s.ctx.stream);
+    return set_error(rc, msg);
+}
+
+// results of a finished slot into the requests' own buffers.  A query that
+// filled its visited table or candidate heap ends early with count -1 (as
+// pk_search reports it): its request fails with PK_ERR_DEVICE, the other
+// requests of the launch are unaffected.
 void complete_slot(pk_server* S, Slot& s, int status) {
     const double t = now_us();
+    const std::string launch_err = status == PK_OK ? std::string() : std::string(pk_last_error());
     for (size_t j = 0; j < s.reqs.size(); j++) {
         Request* r = s.reqs[j];
         const int64_t f = s.first[j];
+        int rs = status;
+        r->err = launch_err;
         if (status == PK_OK) {
             for (int64_t i = 0; i < r->nq; i++) {
                 std::memcpy(r->out_ids + i * r->k, s.h_ids + (f + i) * S->max_k, r->k * 4);
                 std::memcpy(r->out_dists + i * r->k, s.h_d + (f + i) * S->max_k, r->k * 4);
             }
             if (r->out_count) std::memcpy(r->out_count, s.h_cnt + f, r->nq * 4);
+            int64_t bad = 0;
+            for (int64_t i = 0; i < r->nq; i++) bad += s.h_cnt[f + i] < 0;
+            if (bad) {
+                rs = PK_ERR_DEVICE;
+                r->err = std::to_string(bad) +
+                         " queries failed on the device (a visited table or candidate heap filled up)";
+            }
         }
-        r->status = status;
+        r->status = rs;
         r->t_done = t;
         r->done = true;
         S->queries_done += r->nq;
@@ -218,7 +241,7 @@ void dispatcher(pk_server* S) {
                 s.busy = true;
                 S->launches++;
                 int rc = launch_slot(S, s);
-                if (rc) complete_slot(S, s, rc);
+                if (rc) complete_slot(S, s, fail_launch(s, rc));
                 progress = true;
             }
             busy = 0;
@@ -268,8 +291,10 @@ void insert_worker(pk_server* S) {
             i = j;
         }
         if (rc == PK_OK) rc = pk_index_insert(S->h, r->ext_ids.data(), n, 4096, r->flags);
+        const std::string err = rc == PK_OK ? std::string() : std::string(pk_last_error());
         lk.lock();
         r->status = rc;
+        r->err = err;
         r->t_done = now_us();
         r->done = true;
         S->inserts_done++;
@@ -349,6 +374,10 @@ int pk_server_insert(pk_server* S, const int32_t* ext_ids, const void* rows, int
     r->t_submit = now_us();
     {
         std::lock_guard<std::mutex> g(S->mu);
+        if (S->stop) {
+            delete r;
+            return set_error(PK_ERR_STATE, "server is stopping");
+        }
         r->ticket = S->next_ticket++;
         S->by_ticket[r->ticket] = r;
         S->pending_insert.push_back(r);
@@ -364,17 +393,22 @@ int pk_server_wait(pk_server* S, int64_t ticket, int32_t timeout_ms, double* lat
     auto it = S->by_ticket.find(ticket);
     if (it == S->by_ticket.end()) return set_error(PK_ERR_ARG, "unknown ticket");
     Request* r = it->second;
+    if (r->waited) return set_error(PK_ERR_STATE, "another caller is waiting on this ticket");
+    r->waited = true;
     auto pred = [&] { return r->done; };
-    if (timeout_ms < 0)
+    if (timeout_ms < 0) {
         S->cv_done.wait(lk, pred);
-    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
+    } else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) {
+        r->waited = false;  // the ticket stays valid: wait again later
         return set_error(PK_ERR_STATE, "request not finished within the timeout");
+    }
     if (latency_us) *latency_us = r->t_done - r->t_submit;
     if (queue_us) *queue_us = r->t_start - r->t_submit;
-    int status = r->status;
-    S->by_ticket.erase(it);
+    const int status = r->status;
+    const std::string err = r->err;
+    S->by_ticket.erase(ticket);  // by key: submits during the wait may have rehashed the map
     delete r;
-    return status == PK_OK ? PK_OK : set_error(status, "request failed");
+    return status == PK_OK ? PK_OK : set_error(status, "request failed: " + err);
 }
 
 int pk_server_stats(pk_server* S, int64_t* out) {

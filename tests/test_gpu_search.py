@@ -87,6 +87,41 @@ def test_spill_paths_exact(graphs):
     idx.close()
 
 
+def test_full_visited_table_fails_loudly_and_leaves_tables_clean(graphs):
+    """A visited table too small for the search, forced from the host with
+    2^8 slots, ends the query early: the search raises PK_ERR_DEVICE with
+    flag 1.  On this 3000-node graph a query at ef 300 marks at least 485 ids
+    on layer 0 (the oracle's work counters), one at ef 1 at most 159 on any
+    layer, so ef 1 still fits and stays exact.  Each level empties its table
+    through its log also when it fails: a search with larger tables laid over
+    the same memory afterwards gives the reference's results again."""
+    fx, p, base, queries, g = graphs("f32_l2")
+    idx = index_from_oracle(g)
+    idx.set_tuning(0, 12, 0)
+    for k, ef in [(10, 80), (100, 128)]:
+        ids, ds, _ = idx.search(queries, k, ef)
+        np.testing.assert_array_equal(ids, fx[f"ids_k{k}_ef{ef}"])
+    idx.set_tuning(0, 8, 0)  # the same allocation, 256-slot tables packed at its start
+    say("visited table of 256 slots, ef 300")
+    with pytest.raises(abi.PickleError) as e:
+        idx.search(queries, 10, 300)
+    assert e.value.code == abi.ERR_DEVICE
+    assert idx.device_error() == 1
+    ids, ds, cnt = idx.search(queries, 1, 1)
+    oi, od, on = g.search(queries, 1, 1)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(ds, od)
+    np.testing.assert_array_equal(cnt, on)
+    assert idx.device_error() == 0
+    idx.set_tuning(0, 12, 0)
+    for k, ef in [(10, 80), (100, 128)]:
+        tag = f"k{k}_ef{ef}"
+        ids, ds, cnt = idx.search(queries, k, ef)
+        np.testing.assert_array_equal(ids, fx[f"ids_{tag}"], err_msg=tag)
+        np.testing.assert_array_equal(ds, fx[f"dists_{tag}"], err_msg=tag)
+    idx.close()
+
+
 def test_graph_file_round_trip(graphs, tmp_path):
     """pk_index_load_file / save_file on the serializer layout + sidecar."""
     fx, p, base, queries, g = graphs("i8_l2")

@@ -78,3 +78,38 @@ def test_requests_alone_equal_mixed_and_inserts_visible():
     # the loop's post-insert answers equal the eager search of the same graph
     np.testing.assert_array_equal(ids, idx.search(base[n0:n0 + 500], 1, 64)[0])
     idx.close()
+
+
+def test_request_that_overflows_scratch_fails_alone():
+    """A query that fills its visited table (forced from the host with
+    2^8-slot tables) ends early with count -1; its request fails with
+    PK_ERR_DEVICE and the reason, while the other requests, in the same launch
+    or not, return the exact results.  On the 3000-node golden graph a query
+    at ef 1 marks at most 159 ids per level, one at ef 300 at least 485."""
+    from tests import golden_cases as G
+    from tests.gpu_util import index_from_oracle
+
+    fx, p, base, queries = G.graph_case("f32_l2")
+    g = G.oracle_graph(fx, p, base)
+    idx = index_from_oracle(g)
+    idx.set_tuning(0, 8, 0)
+    srv = abi.Server(idx, slots=1, slot_queries=128, max_k=10, max_ef=300)
+    t_ok = srv.submit(queries[:50], 1, 1)
+    t_bad = srv.submit(queries[50:], 10, 300)
+    t_ok2 = srv.submit(queries[50:], 1, 1)
+    say("three requests on 256-slot visited tables")
+    with pytest.raises(abi.PickleError) as e:
+        srv.wait(t_bad)
+    assert e.value.code == abi.ERR_DEVICE and "visited table" in str(e.value)
+    assert (e.value.partial[2] == -1).all()
+    for t, sl in ((t_ok, slice(0, 50)), (t_ok2, slice(50, None))):
+        ids, ds, cnt, _, _ = srv.wait(t)
+        oi, od, on = g.search(queries[sl], 1, 1)
+        np.testing.assert_array_equal(ids, oi)
+        np.testing.assert_array_equal(ds, od)
+        np.testing.assert_array_equal(cnt, on)
+    with pytest.raises(abi.PickleError) as e:
+        srv.wait(t_bad)  # released with its failure
+    assert e.value.code == -1
+    srv.close()
+    idx.close()

@@ -80,7 +80,9 @@ extern "C" int pk_index_load_file(pk_index* h, const char* path) {
     if (!g.ok || nl != nlm || nl < 1 || nl > PK_MAX_LEVELS) return set_error(PK_ERR_IO, "bad layer count");
     for (uint64_t l = 0; l < nl; l++) {
         uint64_t size = g.get<uint64_t>(), n = g.get<uint64_t>(), D = g.get<uint64_t>();
-        if (!g.ok || D == 0 || size != 24 + 8 * n + 8 * n * D) return set_error(PK_ERR_IO, "bad layer header");
+        // bounded before any size arithmetic, so a corrupt header cannot wrap it
+        if (!g.ok || D == 0 || D > 65536 || n > (uint64_t)h->capacity || size != 24 + 8 * n + 8 * n * D)
+            return set_error(PK_ERR_IO, "bad layer header");
         const int32_t* ext = reinterpret_cast<const int32_t*>(g.take(4 * n));
         const int32_t* deg = reinterpret_cast<const int32_t*>(g.take(4 * n));
         const int32_t* adj = reinterpret_cast<const int32_t*>(g.take(4 * n * D));

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -116,7 +117,7 @@ struct pk_index {
     std::vector<uint8_t> levels;  // insert level per external id (InitBuffer)
     bool building = false;
     pk::InsertScratch ins;
-    int enter_level = 0;
+    std::atomic<int> enter_level{0};  // raised by inserts while a serving loop's searches read it
     bool ready = false;
     // search scratch of the handle's own stream
     pk::SearchCtx sc;

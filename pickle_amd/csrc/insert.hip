@@ -808,15 +808,19 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
             if (err)
                 return set_error(PK_ERR_DEVICE, "AddEdge failed on the device (flags " + std::to_string(err) + ")");
         }
-        // SetEntLevel (hnsw.hpp:146-151) and the new sizes become visible
-        for (int64_t k = 0; k < nb; k++) h->enter_level = std::max(h->enter_level, b_lvl[k]);
+        // the new sizes become visible, then SetEntLevel (hnsw.hpp:146-151):
+        // a search launched from another thread that reads the new enter
+        // level also finds the layers it starts on published
         if ((rc = pk::publish_layers(h))) return rc;
+        int top = h->enter_level;
+        for (int64_t k = 0; k < nb; k++) top = std::max(top, b_lvl[k]);
+        h->enter_level = top;
         t = s;
     }
     ICHK(hipStreamSynchronize(h->stream));
     int32_t err = 0;
     ICHK(hipMemcpy(&err, h->ins.err, 4, hipMemcpyDeviceToHost));
     if (err) return set_error(PK_ERR_DEVICE, "insertion failed on the device (flags " + std::to_string(err) + ")");
-    h->ready = h->layers[0].n > 0;
+    if (!h->ready && h->layers[0].n > 0) h->ready = true;  // already set while a serving loop searches
     return PK_OK;
 }

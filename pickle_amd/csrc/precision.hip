@@ -382,6 +382,7 @@ int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, i
                      int32_t* out_ids, float* out_dists, int32_t* out_count, pk_query_counters* counters) {
     if (!h) return set_error(PK_ERR_ARG, "null handle");
     if (nq < 0 || (nq > 0 && (!queries || !out_ids || !out_dists))) return set_error(PK_ERR_ARG, "bad buffers");
+    if (k < 1 || k > k_stride || ef < k) return set_error(PK_ERR_ARG, "need 1 <= k <= k_stride and ef >= k");
     if (nq == 0) return PK_OK;
     void *dq = nullptr, *di = nullptr, *dd = nullptr, *dc = nullptr, *dctr = nullptr;
     int rc;
@@ -411,6 +412,13 @@ int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, i
         (out_count && hipMemcpy(out_count, dc, (size_t)nq * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
         (counters && hipMemcpy(counters, dctr, (size_t)nq * sizeof(pk_query_counters), hipMemcpyDeviceToHost) != hipSuccess))
         return done(set_error(PK_ERR_HIP, "result download failed"));
+    // a query whose traversal failed has count -1; report it as pk_search does
+    int32_t flags = 0;
+    if (hipMemcpy(&flags, h->sc.err, sizeof(flags), hipMemcpyDeviceToHost) != hipSuccess)
+        return done(set_error(PK_ERR_HIP, "failure flags download failed"));
+    if (flags)
+        return done(set_error(PK_ERR_DEVICE, "traversal failed on the device (flags " + std::to_string(flags) +
+                                                 ": 1 = scratch capacity, 2 = bad k/ef)"));
     return done(PK_OK);
 }
 

@@ -129,6 +129,14 @@ def test_rerank_recall_and_errors():
         assert r >= r32 - 0.01
         with pytest.raises(abi.PickleError):
             idx.search_rerank(queries, 10, 5)  # ef < k
+        # 256-slot visited tables, forced from the host: a search at ef 300
+        # keeps 300 distinct ids, so every traversal fills its table
+        idx.set_tuning(0, 8, 0)
+        with pytest.raises(abi.PickleError) as e:
+            idx.search_rerank(queries, 10, 300)
+        assert e.value.code == abi.ERR_DEVICE
+        idx.set_tuning(0, 0, 0)
+        assert rec(idx.search_rerank(queries, 10, 64)[0]) == r  # the tables were left empty
         idx.close()
     with pytest.raises(abi.PickleError):
         src.set_full_rows(base)  # an FP32 index has no separate traversal copy

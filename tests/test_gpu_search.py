@@ -150,6 +150,22 @@ def test_graph_file_round_trip(graphs, tmp_path):
         np.testing.assert_array_equal(a.dist, b.dist)
     idx.close()
 
+    # corrupt files are refused: a first-layer header whose node count makes
+    # the size arithmetic wrap (2^62 nodes of degree 1 "fit" in 24 bytes),
+    # and a file cut short
+    raw = open(path, "rb").read()
+    wrap = raw[:8] + np.array([24, 1 << 62, 1], np.uint64).tobytes() + raw[32:]
+    for name, data in (("wrap.bin", wrap), ("short.bin", raw[: len(raw) // 2])):
+        bad = str(tmp_path / name)
+        open(bad, "wb").write(data)
+        open(bad + ".v2meta", "wb").write(open(path + ".v2meta", "rb").read())
+        idx = abi.Index(np.int8, abi.L2, p["dim"], p["M"], p["efc"], p["n"])
+        idx.set_base(base)
+        with pytest.raises(abi.PickleError) as e:
+            idx.load_file(bad)
+        assert e.value.code == -5, name  # PK_ERR_IO
+        idx.close()
+
 
 def test_duplicate_ids_in_a_row_follow_first_occurrence():
     """A row that names an id twice (possible after the reference's write past

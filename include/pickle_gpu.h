@@ -127,7 +127,9 @@ int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, i
  *   out_dists nq x k_stride distances (float32; -IP for IP), +inf past it
  *   out_count nq result counts (min(k, |W|)), may be NULL
  *   counters  nq per-query counters, may be NULL
- * Host pointers; the call copies in, runs and copies out synchronously. */
+ * Host pointers; the call copies in, runs and copies out synchronously.
+ * ef is at most 630 (the result heap lives in LDS; the reference has no such
+ * bound): a larger ef, like a build ef above 630, returns PK_ERR_ARG. */
 int pk_search(pk_index *h, const void *queries, int64_t nq, const int32_t *k, const int32_t *ef,
               int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t *out_ids, float *out_dists,
               int32_t *out_count, pk_query_counters *counters);

@@ -119,8 +119,9 @@ int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, i
 /* Batched AnnSearch (include/hnsw.hpp:509-541 and SelectSimpleExt :118-125):
  * for each query i, greedy descent through the upper layers, the ef[i]-bounded
  * layer-0 beam search and the ascending top-k[i] with external ids.  Results
- * are bit-identical to the reference for u8/i8 and for f16/f32 whenever the
- * distance order reproduces the reference's (dim % 16 < 8).
+ * are bit-identical to the reference for every dtype: f16/f32 distances sum
+ * in the order of the reference's compiled AVX-512 loop, dim % 16 tail
+ * included.
  *   queries   nq x dim rows of the index dtype
  *   k, ef     per-query arrays (or NULL with k_all / ef_all used for all)
  *   out_ids   nq x k_stride external ids, -1 past the result count

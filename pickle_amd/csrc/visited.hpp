@@ -5,8 +5,9 @@
 // 2^bits 32-bit slots in device memory plus a log of the slots it filled.
 // Test-and-mark is one atomic compare-and-swap per lane against an empty
 // slot, so a chunk of 64 adjacency ids costs one memory round trip; after a
-// search the wave zeroes exactly the slots in its log, so the table is empty
-// again for the next search without any per-search clearing of the table.
+// search the wave zeroes the slots in its log, or, when it marked 1/16 of
+// the slots or more, the whole table with full-line stores, so the table is
+// empty again for the next search.
 //
 // A slot holds (tag << 27) | (id + 1): the tag is the chunk counter mod 32,
 // which tells a lane that finds its own id whether another lane of the same
@@ -29,6 +30,10 @@ struct WaveVisited {
     int peak = 0;        // largest count seen at a clear (diagnostics)
     int full_at = -1;    // count when a probe found the table full (diagnostics)
     uint32_t tag;   // chunk counter
+
+    // marks from which a clear zeroes the whole table instead of the logged
+    // slots: at 1/16 of the slots, most of the table's lines hold a mark
+    __device__ __forceinline__ int full_clear_at() const { return (1 << bits) / 16; }
 
     __device__ __forceinline__ uint32_t home(int id) const {
         return ((uint32_t)id * 0x9E3779B1u) >> (32 - bits);
@@ -112,6 +117,19 @@ struct WaveVisited {
     // Empties the table (the slots this search filled) and orders the clears
     // before the next search's atomics.
     __device__ void clear(int lane) {
+        if (count >= full_clear_at()) {
+            // a search that marked this many slots has touched most of the
+            // table's lines anyway: zero the whole table with full-line
+            // 16-byte stores instead of one scattered store per slot
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            gmem<u32x4>* t = reinterpret_cast<gmem<u32x4>*>(gbl_rw(tab));
+            const int n16 = (1 << bits) / 4;
+            for (int i = lane; i < n16; i += kWave) t[i] = u32x4{0u, 0u, 0u, 0u};
+            peak = count > peak ? count : peak;
+            count = 0;
+            __builtin_amdgcn_s_waitcnt(0);  // stores reach L2 before the next atomics
+            return;
+        }
         // log entries were written by other lanes: read them from L2.  Eight
         // reads go out before their eight stores, so the clear costs a few
         // round trips rather than one per 64 slots.

@@ -130,7 +130,12 @@ int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, i
  *   counters  nq per-query counters, may be NULL
  * Host pointers; the call copies in, runs and copies out synchronously.
  * ef is at most 630 (the result heap lives in LDS; the reference has no such
- * bound): a larger ef, like a build ef above 630, returns PK_ERR_ARG. */
+ * bound): a larger ef, like a build ef above 630, returns PK_ERR_ARG.
+ * Queries that fill their visited table are searched again with larger
+ * tables (up to 2^20 slots; results do not depend on the size), so this call
+ * returns PK_ERR_DEVICE only if that still fails or captured search graphs
+ * keep the scratch from growing.  The pk_search_device* calls report such
+ * queries through their counts and pk_device_error instead. */
 int pk_search(pk_index *h, const void *queries, int64_t nq, const int32_t *k, const int32_t *ef,
               int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t *out_ids, float *out_dists,
               int32_t *out_count, pk_query_counters *counters);

@@ -837,6 +837,54 @@ int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, co
     int32_t errflag = 0;
     HIPCHK(hipMemcpyAsync(&errflag, h->sc.err, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    // Queries that filled their visited table (flag 1 alone) are searched
+    // again with a table twice the size, up to 2^20 slots; the table size
+    // does not change results.  Not while captured search graphs pin the
+    // scratch.
+    constexpr int kVisitedFull = 1;
+    int vb = h->tuning.vbits > 0 ? h->tuning.vbits : default_vbits(ef_max, h->capacity);
+    while (errflag == kVisitedFull && vb < 20 && h->graphs_alive == 0) {
+        vb++;
+        std::vector<int64_t> bad;
+        for (int64_t i = 0; i < nq; i++)
+            if (cdst[i] < 0) bad.push_back(i);
+        const int64_t nb = (int64_t)bad.size();
+        if (nb == 0) break;
+        const size_t qb = (size_t)h->qbytes;
+        std::vector<uint8_t> q2((size_t)nb * qb);
+        std::vector<int32_t> k2(k ? nb : 0), e2(ef ? nb : 0);
+        for (int64_t j = 0; j < nb; j++) {
+            std::memcpy(q2.data() + j * qb, static_cast<const uint8_t*>(queries) + bad[j] * qb, qb);
+            if (k) k2[j] = k[bad[j]];
+            if (ef) e2[j] = ef[bad[j]];
+        }
+        HIPCHK(hipMemcpyAsync(dq, q2.data(), q2.size(), hipMemcpyHostToDevice, h->stream));
+        if (k) HIPCHK(hipMemcpyAsync(dk, k2.data(), (size_t)nb * 4, hipMemcpyHostToDevice, h->stream));
+        if (ef) HIPCHK(hipMemcpyAsync(def, e2.data(), (size_t)nb * 4, hipMemcpyHostToDevice, h->stream));
+        const SearchTuning saved = h->tuning;
+        h->tuning.vbits = vb;
+        rc = pk_search_device_ex(h, dq, nb, (const int32_t*)dk, (const int32_t*)def, k_all, ef_all, ef_max, k_stride,
+                                 (int32_t*)dids, (float*)dd, (int32_t*)dc, (pk_query_counters*)dctr);
+        h->tuning = saved;
+        if (rc) return rc;
+        std::vector<int32_t> ids2((size_t)nb * k_stride), cnt2(nb);
+        std::vector<float> d2((size_t)nb * k_stride);
+        std::vector<pk_query_counters> ctr2(counters ? nb : 0);
+        HIPCHK(hipMemcpyAsync(ids2.data(), dids, ids2.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(d2.data(), dd, d2.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(cnt2.data(), dc, (size_t)nb * 4, hipMemcpyDeviceToHost, h->stream));
+        if (counters)
+            HIPCHK(hipMemcpyAsync(ctr2.data(), dctr, (size_t)nb * sizeof(pk_query_counters), hipMemcpyDeviceToHost,
+                                  h->stream));
+        HIPCHK(hipMemcpyAsync(&errflag, h->sc.err, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t j = 0; j < nb; j++) {
+            std::memcpy(out_ids + bad[j] * k_stride, ids2.data() + j * k_stride, (size_t)k_stride * 4);
+            std::memcpy(out_dists + bad[j] * k_stride, d2.data() + j * k_stride, (size_t)k_stride * 4);
+            cdst[bad[j]] = cnt2[j];
+            if (counters) counters[bad[j]] = ctr2[j];
+        }
+    }
     if (errflag) {
         int64_t bad = 0;
         for (int64_t i = 0; i < nq; i++) bad += cdst[i] < 0;

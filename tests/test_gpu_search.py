@@ -89,12 +89,14 @@ def test_spill_paths_exact(graphs):
 
 def test_full_visited_table_fails_loudly_and_leaves_tables_clean(graphs):
     """A visited table too small for the search, forced from the host with
-    2^8 slots, ends the query early: the search raises PK_ERR_DEVICE with
-    flag 1.  On this 3000-node graph a query at ef 300 marks at least 485 ids
-    on layer 0 (the oracle's work counters), one at ef 1 at most 159 on any
-    layer, so ef 1 still fits and stays exact.  Each level empties its table
-    through its log also when it fails: a search with larger tables laid over
-    the same memory afterwards gives the reference's results again."""
+    2^8 slots, ends the query early: the device call reports flag 1 and
+    count -1.  On this 3000-node graph a query at ef 300 marks at least 485
+    ids on layer 0 (the oracle's work counters), one at ef 1 at most 159 on
+    any layer, so ef 1 still fits and stays exact.  The host call pk_search
+    searches the failed queries again with larger tables and returns the
+    reference's results.  Each level empties its table also when it fails: a
+    search with larger tables laid over the same memory afterwards gives the
+    reference's results again."""
     fx, p, base, queries, g = graphs("f32_l2")
     idx = index_from_oracle(g)
     idx.set_tuning(0, 12, 0)
@@ -103,10 +105,20 @@ def test_full_visited_table_fails_loudly_and_leaves_tables_clean(graphs):
         np.testing.assert_array_equal(ids, fx[f"ids_k{k}_ef{ef}"])
     idx.set_tuning(0, 8, 0)  # the same allocation, 256-slot tables packed at its start
     say("visited table of 256 slots, ef 300")
-    with pytest.raises(abi.PickleError) as e:
-        idx.search(queries, 10, 300)
-    assert e.value.code == abi.ERR_DEVICE
+    nq = queries.shape[0]
+    dq = abi.DeviceBuffer.from_array(np.ascontiguousarray(queries))
+    d_ids, d_d, d_c = abi.DeviceBuffer(nq * 10 * 4), abi.DeviceBuffer(nq * 10 * 4), abi.DeviceBuffer(nq * 4)
+    idx.search_device(dq, nq, 10, 300, 10, d_ids, d_d, d_c)
+    idx.sync()
     assert idx.device_error() == 1
+    assert (d_c.to_array(np.int32, (nq,)) == -1).all()
+    ids, ds, cnt = idx.search(queries, 10, 300)  # failed queries rerun with 2^9, 2^10, ... slots
+    oi, od, on = g.search(queries, 10, 300)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(ds, od)
+    np.testing.assert_array_equal(cnt, on)
+    for b in (dq, d_ids, d_d, d_c):
+        b.free()
     ids, ds, cnt = idx.search(queries, 1, 1)
     oi, od, on = g.search(queries, 1, 1)
     np.testing.assert_array_equal(ids, oi)

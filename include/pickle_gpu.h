@@ -196,9 +196,11 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
 /* Queues nq queries (host rows, copied at once); results go to the caller's
  * out_ids / out_dists (nq x k) and out_count (nq, may be NULL) when the
- * request completes, so those buffers must outlive the request.  A query that
- * fills its visited table or candidate heap gets count -1 and its request
- * fails with PK_ERR_DEVICE; the other requests of its launch do not. */
+ * request completes, so those buffers must outlive the request.  Queries that
+ * fill their visited table run again with tables twice the size, up to 2^20
+ * slots, before their request completes (results do not depend on the size);
+ * past that the request fails with PK_ERR_DEVICE and count -1 for those
+ * queries.  Other requests are not held back. */
 int pk_server_submit(pk_server *s, const void *queries, int64_t nq, int32_t k, int32_t ef,
                      int32_t *out_ids, float *out_dists, int32_t *out_count, int64_t *ticket);
 /* Queues Insert of n rows (host, by external id; copied at once). */

@@ -448,36 +448,46 @@ void free_search_ctx(SearchCtx& c) {
     dfree(c.vlog);
     dfree(c.qcounter);
     dfree(c.err);
-    c.waves = c.vbits = 0;
+    c.v_slots = c.cg_entries = 0;
 }
 
+// Scratch for a launch of `waves` waves with 2^vbits-slot visited tables.
+// Each buffer is kept as large as the largest launch so far needed, counted
+// in slots / entries whatever the layout: the tables are all zero between
+// searches, so any later layout fits inside them, and a launch of a few
+// waves with large tables (the queries searched again after filling theirs)
+// needs no more memory than a full launch with small ones.
 static int ensure_scratch(pk_index* h, SearchCtx& c, int waves, int vbits) {
-    if (waves <= c.waves && vbits <= c.vbits) return PK_OK;
+    // the candidate heap only holds first-visited ids, so with room for
+    // 2^vbits entries per wave it cannot fill before the visited table does
+    const int cg_stride = std::max(h->cg_cap, 1 << vbits);
+    const size_t need_v = (size_t)waves << vbits, need_cg = (size_t)waves * cg_stride;
+    if (need_v <= c.v_slots && need_cg <= c.cg_entries && c.qcounter && c.err) {
+        c.cg_cap = cg_stride;
+        return PK_OK;
+    }
     if (&c == &h->sc && h->graphs_alive)
         return fail(PK_ERR_STATE, "this search needs more scratch than the captured search graphs use; destroy them "
                                   "first");
-    waves = std::max(waves, c.waves);
-    vbits = std::max(vbits, c.vbits);
     HIPCHK(hipStreamSynchronize(c.stream));
-    dfree(c.cglob);
-    dfree(c.vtab);
-    dfree(c.vlog);
-    c.waves = 0;
-    c.vbits = 0;
     int rc;
-    size_t slots = (size_t)waves << vbits;
-    // the candidate heap only holds first-visited ids, so with room for
-    // 2^vbits entries it cannot fill before the visited table does
-    const int cg_cap = std::max(h->cg_cap, 1 << vbits);
-    if ((rc = dalloc(&c.cglob, (size_t)waves * cg_cap))) return rc;
-    if ((rc = dalloc(&c.vtab, slots))) return rc;
-    if ((rc = dalloc(&c.vlog, slots))) return rc;
-    HIPCHK(hipMemsetAsync(c.vtab, 0, slots * sizeof(uint32_t), c.stream));
+    if (need_cg > c.cg_entries) {
+        dfree(c.cglob);
+        c.cg_entries = 0;
+        if ((rc = dalloc(&c.cglob, need_cg))) return rc;
+        c.cg_entries = need_cg;
+    }
+    if (need_v > c.v_slots) {
+        dfree(c.vtab);
+        dfree(c.vlog);
+        c.v_slots = 0;
+        if ((rc = dalloc(&c.vtab, need_v)) || (rc = dalloc(&c.vlog, need_v))) return rc;
+        HIPCHK(hipMemsetAsync(c.vtab, 0, need_v * sizeof(uint32_t), c.stream));
+        c.v_slots = need_v;
+    }
     if (!c.qcounter && (rc = dalloc(&c.qcounter, 1))) return rc;
     if (!c.err && (rc = dalloc(&c.err, 1))) return rc;
-    c.waves = waves;
-    c.vbits = vbits;
-    c.cg_cap = cg_cap;
+    c.cg_cap = cg_stride;
     return PK_OK;
 }
 }  // namespace pk
@@ -516,7 +526,7 @@ static int search_class(const pk_index* h) {
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
-                       SearchPlan* P) {
+                       SearchPlan* P, int vbits_override = 0) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
     if (!h->ready) return fail(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
     if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return fail(PK_ERR_ARG, "bad buffers");
@@ -554,7 +564,7 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     int wcap = result_capacity_for(ef_max);
     if (wcap < 0) return fail(PK_ERR_ARG, "ef_max %d above the largest supported ef (630)", ef_max);
     int ccap = t.ccap > 0 ? t.ccap : (ef_max <= 64 ? 512 : 1024);
-    int vbits = t.vbits > 0 ? t.vbits : default_vbits(ef_max, h->capacity);
+    int vbits = vbits_override > 0 ? vbits_override : t.vbits > 0 ? t.vbits : default_vbits(ef_max, h->capacity);
     if (h->capacity > kVisitMaxId) return fail(PK_ERR_CAPACITY, "search supports at most %lld ids", (long long)kVisitMaxId);
     size_t per_wave = (size_t)wcap * 8 + (size_t)ccap * 8 + 64 * 4 * 2 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
@@ -568,6 +578,7 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     int waves = grid * (kSearchBlock / 64);
     int rc = ensure_scratch(h, c, waves, vbits);
     if (rc) return rc;
+    c.last_vbits = vbits;
     a.cglob = c.cglob;
     a.vtab = c.vtab;
     a.vlog = c.vlog;
@@ -595,11 +606,11 @@ static int enqueue_search(pk_index* h, SearchCtx& c, const SearchPlan& P) {
 
 int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
               int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
-              int32_t* d_out_count, pk_query_counters* d_counters) {
+              int32_t* d_out_count, pk_query_counters* d_counters, int vbits_override) {
     if (h && nq == 0 && d_out_ids) return PK_OK;
     SearchPlan P;
     int rc = plan_search(h, c, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids, d_out_dists,
-                         d_out_count, d_counters, &P);
+                         d_out_count, d_counters, &P, vbits_override);
     if (rc) return rc;
     if (nq == 0) return PK_OK;
     return enqueue_search(h, c, P);

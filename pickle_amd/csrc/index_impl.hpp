@@ -62,9 +62,10 @@ struct SearchCtx {
     uint64_t* cglob = nullptr;
     uint32_t* vtab = nullptr;
     int32_t* vlog = nullptr;
-    int waves = 0;
-    int vbits = 0;
-    int cg_cap = 0;  // candidate-heap entries per wave in device memory (>= 2^vbits)
+    size_t v_slots = 0;     // visited-table / log slots allocated (any waves x 2^vbits layout that fits)
+    size_t cg_entries = 0;  // candidate-heap spill entries allocated
+    int cg_cap = 0;  // candidate-heap entries per wave of the last planned launch (>= 2^vbits)
+    int last_vbits = 0;  // visited-table size of the last search planned on this context
     int32_t* qcounter = nullptr;
     int32_t* err = nullptr;
 };
@@ -93,10 +94,11 @@ int gather_layer_vectors(pk_index* h, int level);
 hipError_t upload_rows(uint8_t* dst, size_t dst_pitch, const void* src, size_t row_bytes, int64_t n,
                        hipStream_t s);
 // one batched search on a given context (its stream and scratch); the
-// handle's own calls use h->sc
+// handle's own calls use h->sc.  vbits_override > 0 sets the visited-table
+// size instead of the handle's tuning.
 int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
               int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
-              int32_t* d_out_count, pk_query_counters* d_counters);
+              int32_t* d_out_count, pk_query_counters* d_counters, int vbits_override = 0);
 void free_search_ctx(SearchCtx& c);
 }  // namespace pk
 

@@ -59,6 +59,13 @@ struct Request {
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
     bool waited = false;  // a caller is in pk_server_wait on it
+    int vbits = 0;        // visited-table size of its next run (0 = the handle's tuning)
+    // an internal rerun of some rows of `parent`: their indices there, and
+    // the results until they are copied into the parent's buffers
+    Request* parent = nullptr;
+    std::vector<int64_t> rows;
+    std::vector<int32_t> res_ids, res_count;
+    std::vector<float> res_dists;
     int status = PK_OK;
     std::string err;  // the failure's message (set on the thread that saw it)
     // insert request
@@ -80,6 +87,7 @@ struct Slot {
     std::vector<int64_t> first;  // first query row of each request in the launch
     bool busy = false;
     int64_t launched = 0;
+    int vbits = 0;  // visited-table size of the launch (0 = the handle's tuning)
 };
 
 }  // namespace
@@ -152,7 +160,7 @@ int launch_slot(pk_server* S, Slot& s) {
     SCHK(hipMemcpyAsync(s.d_k, s.h_k, nq * 4, hipMemcpyHostToDevice, st));
     SCHK(hipMemcpyAsync(s.d_ef, s.h_ef, nq * 4, hipMemcpyHostToDevice, st));
     int rc = search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, S->max_ef, S->max_k, s.d_ids, s.d_d, s.d_cnt,
-                       nullptr);
+                       nullptr, s.vbits);
     if (rc) return rc;
     SCHK(hipMemcpyAsync(s.h_ids, s.d_ids, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
     SCHK(hipMemcpyAsync(s.h_d, s.d_d, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
@@ -170,37 +178,90 @@ int fail_launch(Slot& s, int rc) {
     return set_error(rc, msg);
 }
 
-// results of a finished slot into the requests' own buffers.  A query that
-// filled its visited table or candidate heap ends early with count -1 (as
-// pk_search reports it): its request fails with PK_ERR_DEVICE, the other
-// requests of the launch are unaffected.
+// A finished top-level request, or one whose rerun of its failed rows ended.
+void finish_request(pk_server* S, Request* r, int status, const std::string& err, double t) {
+    r->status = status;
+    r->err = err;
+    r->t_done = t;
+    r->done = true;
+    S->queries_done += r->nq;
+}
+
+// The end of a rerun (child request) of some rows of its parent: its
+// results go into the parent's rows, and the parent completes.
+void finish_child(pk_server* S, Request* c, int status, const std::string& err, double t) {
+    Request* p = c->parent;
+    for (size_t j = 0; j < c->rows.size(); j++) {
+        const int64_t i = c->rows[j];
+        std::memcpy(p->out_ids + i * p->k, c->out_ids + (int64_t)j * c->k, c->k * 4);
+        std::memcpy(p->out_dists + i * p->k, c->out_dists + (int64_t)j * c->k, c->k * 4);
+        if (p->out_count) p->out_count[i] = c->out_count[j];
+    }
+    finish_request(S, p, status, err, t);
+    delete c;
+}
+
+// Results of a finished slot into the requests' own buffers.  A query that
+// filled its visited table ends early with count -1 (k and ef were checked
+// at submit, and the candidate heap cannot fill first).  Its rows run again
+// as one internal request, ahead of the waiting ones, with a table twice the
+// size, up to 2^20 slots (the size does not change results); the request
+// completes when they do, and fails with PK_ERR_DEVICE past that size.  The
+// other requests of the launch are unaffected.
 void complete_slot(pk_server* S, Slot& s, int status) {
     const double t = now_us();
     const std::string launch_err = status == PK_OK ? std::string() : std::string(pk_last_error());
+    const size_t qb = (size_t)S->h->qbytes;
+    std::vector<Request*> rerun;
     for (size_t j = 0; j < s.reqs.size(); j++) {
         Request* r = s.reqs[j];
         const int64_t f = s.first[j];
         int rs = status;
-        r->err = launch_err;
+        std::string err = launch_err;
         if (status == PK_OK) {
             for (int64_t i = 0; i < r->nq; i++) {
                 std::memcpy(r->out_ids + i * r->k, s.h_ids + (f + i) * S->max_k, r->k * 4);
                 std::memcpy(r->out_dists + i * r->k, s.h_d + (f + i) * S->max_k, r->k * 4);
             }
             if (r->out_count) std::memcpy(r->out_count, s.h_cnt + f, r->nq * 4);
-            int64_t bad = 0;
-            for (int64_t i = 0; i < r->nq; i++) bad += s.h_cnt[f + i] < 0;
-            if (bad) {
+            std::vector<int64_t> bad;
+            for (int64_t i = 0; i < r->nq; i++)
+                if (s.h_cnt[f + i] < 0) bad.push_back(i);
+            if (!bad.empty() && s.ctx.last_vbits < 20) {
+                Request* c = r;  // a rerun that failed again runs again, whole
+                if (!r->parent) {
+                    c = new Request;
+                    c->parent = r;
+                    c->rows = bad;
+                    c->nq = (int64_t)bad.size();
+                    c->k = r->k;
+                    c->ef = r->ef;
+                    c->queries.resize((size_t)c->nq * qb);
+                    for (int64_t b = 0; b < c->nq; b++)
+                        std::memcpy(c->queries.data() + b * qb, r->queries.data() + bad[b] * qb, qb);
+                    c->res_ids.resize((size_t)c->nq * c->k);
+                    c->res_dists.resize((size_t)c->nq * c->k);
+                    c->res_count.resize((size_t)c->nq);
+                    c->out_ids = c->res_ids.data();
+                    c->out_dists = c->res_dists.data();
+                    c->out_count = c->res_count.data();
+                }
+                c->vbits = s.ctx.last_vbits + 1;
+                rerun.push_back(c);
+                continue;
+            }
+            if (!bad.empty()) {
                 rs = PK_ERR_DEVICE;
-                r->err = std::to_string(bad) +
-                         " queries failed on the device (a visited table or candidate heap filled up)";
+                err = std::to_string(bad.size()) +
+                      " queries failed on the device (a visited table of 2^20 slots filled up)";
             }
         }
-        r->status = rs;
-        r->t_done = t;
-        r->done = true;
-        S->queries_done += r->nq;
+        if (r->parent)
+            finish_child(S, r, rs, err, t);
+        else
+            finish_request(S, r, rs, err, t);
     }
+    for (auto it = rerun.rbegin(); it != rerun.rend(); ++it) S->pending.push_front(*it);
     s.reqs.clear();
     s.busy = false;
 }
@@ -225,19 +286,23 @@ void dispatcher(pk_server* S) {
         if (progress) S->cv_done.notify_all();
         if (S->stop && S->pending.empty() && busy == 0) break;
         {
-            // admission: free slots take waiting searches in arrival order;
-            // searches are not held behind inserts
+            // admission: free slots take waiting searches in arrival order
+            // (reruns first), one visited-table size per launch; searches are
+            // not held behind inserts
             for (Slot& s : S->slots) {
                 if (s.busy || S->pending.empty()) continue;
                 int64_t nq = 0;
-                while (!S->pending.empty() && nq + S->pending.front()->nq <= S->slot_queries) {
+                const int vb = S->pending.front()->vbits;
+                while (!S->pending.empty() && S->pending.front()->vbits == vb &&
+                       nq + S->pending.front()->nq <= S->slot_queries) {
                     Request* r = S->pending.front();
                     S->pending.pop_front();
-                    r->t_start = now_us();
+                    if (r->t_start == 0) r->t_start = now_us();
                     nq += r->nq;
                     s.reqs.push_back(r);
                 }
                 if (s.reqs.empty()) continue;
+                s.vbits = vb;
                 s.busy = true;
                 S->launches++;
                 int rc = launch_slot(S, s);

@@ -87,7 +87,7 @@ def test_spill_paths_exact(graphs):
     idx.close()
 
 
-def test_full_visited_table_fails_loudly_and_leaves_tables_clean(graphs):
+def test_full_visited_table_is_reported_rerun_and_left_clean(graphs):
     """A visited table too small for the search, forced from the host with
     2^8 slots, ends the query early: the device call reports flag 1 and
     count -1.  On this 3000-node graph a query at ef 300 marks at least 485
@@ -112,11 +112,15 @@ def test_full_visited_table_fails_loudly_and_leaves_tables_clean(graphs):
     idx.sync()
     assert idx.device_error() == 1
     assert (d_c.to_array(np.int32, (nq,)) == -1).all()
-    ids, ds, cnt = idx.search(queries, 10, 300)  # failed queries rerun with 2^9, 2^10, ... slots
-    oi, od, on = g.search(queries, 10, 300)
+    # failed queries rerun with 2^9, 2^10, ... slots; their counters come back too
+    ids, ds, cnt, ctr = idx.search(queries, 10, 300, counters=True)
+    oi, od, on, octr = g.search(queries, 10, 300, counters=True)
     np.testing.assert_array_equal(ids, oi)
     np.testing.assert_array_equal(ds, od)
     np.testing.assert_array_equal(cnt, on)
+    for name in ("hops", "neighbors", "dists"):
+        assert int(ctr[name][:, 0].sum()) == octr[name][0], name  # layer 0
+        assert int(ctr[name].sum()) == sum(octr[name]), name
     for b in (dq, d_ids, d_d, d_c):
         b.free()
     ids, ds, cnt = idx.search(queries, 1, 1)

@@ -192,7 +192,7 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
  * concurrent Insert and AnnSearch, a search running meanwhile may see part
  * of an insert, and every search submitted after the insert's
  * pk_server_wait returns sees all of it.  All calls are thread-safe.  While a server exists
- * the index's layers cannot be replaced. */
+ * the index's layers cannot be replaced; the handle's own searches still run. */
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
 /* Queues nq queries (host rows, copied at once); results go to the caller's
  * out_ids / out_dists (nq x k) and out_count (nq, may be NULL) when the

@@ -242,7 +242,8 @@ int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capa
 }
 
 int pk_index_destroy(pk_index* h) {
-    if (h && h->graphs_alive) return fail(PK_ERR_STATE, "destroy the index's search graphs first");
+    if (h && (h->graphs_alive || h->servers_alive))
+        return fail(PK_ERR_STATE, "destroy the index's search graphs and serving loops first");
     if (!h) return PK_OK;
     (void)hipStreamSynchronize(h->stream);
     for (auto& L : h->layers) free_layer(L);
@@ -335,7 +336,8 @@ int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_le
                         const int32_t* degree, const int16_t* heur, const int32_t* adj, const float* dist,
                         const uint8_t* has_vector) {
     if (!h || !ext || !degree || !adj || !has_vector) return fail(PK_ERR_ARG, "null argument");
-    if (h->graphs_alive) return fail(PK_ERR_STATE, "destroy the search graphs before replacing layers");
+    if (h->graphs_alive || h->servers_alive)
+        return fail(PK_ERR_STATE, "destroy the search graphs and serving loops before replacing layers");
     if (level < 0 || level >= PK_MAX_LEVELS) return fail(PK_ERR_ARG, "level %d out of range", level);
     if (n < 0 || n > h->capacity || D < 1 || adj_len < n * D) return fail(PK_ERR_ARG, "bad layer shape");
     if (!h->base) return fail(PK_ERR_STATE, "set the base rows before loading layers");

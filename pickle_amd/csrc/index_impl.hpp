@@ -137,7 +137,8 @@ struct pk_index {
     float* q_inv_scale = nullptr;
     bool has_quantizer = false;
     pk::IoBuf rr[5];  // re-rank scratch: converted queries, candidate ids / dists / counts, counter
-    int graphs_alive = 0;  // captured search graphs hold pointers into scratch and layers
+    int graphs_alive = 0;   // captured search graphs hold pointers into scratch and layers
+    int servers_alive = 0;  // serving loops read the layers (their scratch is their own)
 };
 
 struct pk_search_graph {

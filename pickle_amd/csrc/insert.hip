@@ -627,7 +627,8 @@ extern "C" int pk_sample_levels(int64_t n, int M, uint64_t seed, uint8_t* out) {
 extern "C" int pk_index_init_build(pk_index* h, const uint8_t* levels) {
     if (!h || !levels) return set_error(PK_ERR_ARG, "null argument");
     if (!h->base) return set_error(PK_ERR_STATE, "set the base rows before init_build");
-    if (h->graphs_alive) return set_error(PK_ERR_STATE, "destroy the search graphs before init_build");
+    if (h->graphs_alive || h->servers_alive)
+        return set_error(PK_ERR_STATE, "destroy the search graphs and serving loops before init_build");
     if (2 * h->M > kTopKMax) return set_error(PK_ERR_ARG, "M above 64 is not supported by the GPU build");
     h->levels.assign(levels, levels + h->capacity);
     int maxl = 0;

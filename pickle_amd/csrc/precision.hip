@@ -296,7 +296,8 @@ int pk_index_copy_graph(pk_index* dst, pk_index* src) {
     if (dst->capacity != src->capacity || dst->dim != src->dim || dst->M != src->M)
         return set_error(PK_ERR_ARG, "indexes differ in capacity, dim or M");
     if (!dst->base) return set_error(PK_ERR_STATE, "set the destination's rows first");
-    if (dst->graphs_alive) return set_error(PK_ERR_STATE, "destroy the destination's search graphs first");
+    if (dst->graphs_alive || dst->servers_alive)
+        return set_error(PK_ERR_STATE, "destroy the destination's search graphs and serving loops first");
     PCHK(hipStreamSynchronize(src->stream));
     PCHK(hipStreamSynchronize(dst->stream));
     int rc;

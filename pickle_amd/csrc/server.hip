@@ -392,7 +392,7 @@ int pk_server_create(pk_index* h, int slots, int slot_queries, int max_k, int ma
         }
     }
     (void)hipGetDevice(&S->device);
-    h->graphs_alive++;  // the slots' launches read the layers: no reallocation meanwhile
+    h->servers_alive++;  // the slots' launches read the layers: no reallocation meanwhile
     S->th = std::thread(dispatcher, S);
     S->th_insert = std::thread(insert_worker, S);
     *out = S;
@@ -501,7 +501,7 @@ int pk_server_destroy(pk_server* S) {
         free_slot(s);
     }
     for (auto& kv : S->by_ticket) delete kv.second;
-    S->h->graphs_alive--;
+    S->h->servers_alive--;
     delete S;
     return PK_OK;
 }

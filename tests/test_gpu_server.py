@@ -95,6 +95,12 @@ def test_request_that_overflows_scratch_reruns_alone():
     idx = index_from_oracle(g)
     idx.set_tuning(0, 8, 0)
     srv = abi.Server(idx, slots=1, slot_queries=128, max_k=10, max_ef=300)
+    # the handle's own search still runs while a server holds the index (its
+    # scratch is first allocated here)
+    ids, ds, _ = idx.search(queries[:10], 10, 40)
+    oi, od, _ = g.search(queries[:10], 10, 40)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(ds, od)
     t_ok = srv.submit(queries[:50], 1, 1)
     t_big = srv.submit(queries[50:], 10, 300)
     t_ok2 = srv.submit(queries[50:], 1, 1)

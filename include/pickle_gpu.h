@@ -299,9 +299,10 @@ int pk_index_copy_graph(pk_index *dst, pk_index *src);
 int pk_search_rerank_device(pk_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
                             int32_t k_stride, int32_t *d_out_ids, float *d_out_dists, int32_t *d_out_count,
                             pk_query_counters *d_counters);
-/* The same with host buffers (copies in, runs, copies out); a failed
- * traversal returns PK_ERR_DEVICE, as pk_search does, after the outputs are
- * written (count -1 for the failed queries). */
+/* The same with host buffers (copies in, runs, copies out).  As in
+ * pk_search, queries whose traversal fills its visited table run again with
+ * larger tables (up to 2^20 slots); PK_ERR_DEVICE only if that still fails,
+ * with count -1 for those queries. */
 int pk_search_rerank(pk_index *h, const float *queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,
                      int32_t *out_ids, float *out_dists, int32_t *out_count, pk_query_counters *counters);
 

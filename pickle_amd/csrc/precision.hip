@@ -18,6 +18,7 @@
 // values.  FP16 copy: round to nearest even.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -407,16 +408,59 @@ int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, i
                                  static_cast<float*>(dd), static_cast<int32_t*>(dc),
                                  static_cast<pk_query_counters*>(dctr));
     if (rc) return done(rc);
+    std::vector<int32_t> cnt((size_t)nq);
     if (hipStreamSynchronize(h->stream) != hipSuccess ||
         hipMemcpy(out_ids, di, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(out_dists, dd, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-        (out_count && hipMemcpy(out_count, dc, (size_t)nq * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+        hipMemcpy(cnt.data(), dc, (size_t)nq * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         (counters && hipMemcpy(counters, dctr, (size_t)nq * sizeof(pk_query_counters), hipMemcpyDeviceToHost) != hipSuccess))
         return done(set_error(PK_ERR_HIP, "result download failed"));
-    // a query whose traversal failed has count -1; report it as pk_search does
     int32_t flags = 0;
     if (hipMemcpy(&flags, h->sc.err, sizeof(flags), hipMemcpyDeviceToHost) != hipSuccess)
         return done(set_error(PK_ERR_HIP, "failure flags download failed"));
+    // Queries whose traversal filled its visited table (flag 1 alone, count
+    // -1) run again with tables twice the size, up to 2^20 slots, as in
+    // pk_search; the size does not change results.  Not while captured
+    // search graphs pin the scratch.
+    const int dim = h->dim;
+    int vb = h->sc.last_vbits;
+    while (flags == 1 && vb < 20 && h->graphs_alive == 0) {
+        vb++;
+        std::vector<int64_t> bad;
+        for (int64_t i = 0; i < nq; i++)
+            if (cnt[i] < 0) bad.push_back(i);
+        const int64_t nb = (int64_t)bad.size();
+        if (nb == 0) break;
+        std::vector<float> q2((size_t)nb * dim);
+        for (int64_t j = 0; j < nb; j++) std::memcpy(&q2[(size_t)j * dim], queries + bad[j] * dim, (size_t)dim * 4);
+        if (hipMemcpy(dq, q2.data(), q2.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return done(set_error(PK_ERR_HIP, "query upload failed"));
+        const int saved = h->tuning.vbits;
+        h->tuning.vbits = vb;
+        rc = pk_search_rerank_device(h, static_cast<const float*>(dq), nb, k, ef, k_stride, static_cast<int32_t*>(di),
+                                     static_cast<float*>(dd), static_cast<int32_t*>(dc),
+                                     static_cast<pk_query_counters*>(dctr));
+        h->tuning.vbits = saved;
+        if (rc) return done(rc);
+        std::vector<int32_t> ids2((size_t)nb * k_stride), cnt2((size_t)nb);
+        std::vector<float> d2((size_t)nb * k_stride);
+        std::vector<pk_query_counters> ctr2(counters ? nb : 0);
+        if (hipStreamSynchronize(h->stream) != hipSuccess ||
+            hipMemcpy(ids2.data(), di, ids2.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(d2.data(), dd, d2.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(cnt2.data(), dc, cnt2.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            (counters && hipMemcpy(ctr2.data(), dctr, (size_t)nb * sizeof(pk_query_counters), hipMemcpyDeviceToHost) !=
+                             hipSuccess) ||
+            hipMemcpy(&flags, h->sc.err, sizeof(flags), hipMemcpyDeviceToHost) != hipSuccess)
+            return done(set_error(PK_ERR_HIP, "result download failed"));
+        for (int64_t j = 0; j < nb; j++) {
+            std::memcpy(out_ids + bad[j] * k_stride, &ids2[(size_t)j * k_stride], (size_t)k_stride * 4);
+            std::memcpy(out_dists + bad[j] * k_stride, &d2[(size_t)j * k_stride], (size_t)k_stride * 4);
+            cnt[bad[j]] = cnt2[j];
+            if (counters) counters[bad[j]] = ctr2[j];
+        }
+    }
+    if (out_count) std::memcpy(out_count, cnt.data(), (size_t)nq * 4);
     if (flags)
         return done(set_error(PK_ERR_DEVICE, "traversal failed on the device (flags " + std::to_string(flags) +
                                                  ": 1 = scratch capacity, 2 = bad k/ef)"));

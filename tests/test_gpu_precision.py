@@ -130,11 +130,22 @@ def test_rerank_recall_and_errors():
         with pytest.raises(abi.PickleError):
             idx.search_rerank(queries, 10, 5)  # ef < k
         # 256-slot visited tables, forced from the host: a search at ef 300
-        # keeps 300 distinct ids, so every traversal fills its table
+        # keeps 300 distinct ids, so every traversal fills its table.  The
+        # device call reports it; the host call runs those queries again with
+        # larger tables and returns what the default tables give.
+        want = idx.search_rerank(queries, 10, 300)
         idx.set_tuning(0, 8, 0)
-        with pytest.raises(abi.PickleError) as e:
-            idx.search_rerank(queries, 10, 300)
-        assert e.value.code == abi.ERR_DEVICE
+        nq = queries.shape[0]
+        dq = abi.DeviceBuffer.from_array(np.ascontiguousarray(queries, np.float32))
+        d_ids, d_d, d_c = abi.DeviceBuffer(nq * 40), abi.DeviceBuffer(nq * 40), abi.DeviceBuffer(nq * 4)
+        idx.search_rerank_device(dq, nq, 10, 300, 10, d_ids, d_d, d_c)
+        idx.sync()
+        assert idx.device_error() == 1
+        assert (d_c.to_array(np.int32, (nq,)) == -1).all()
+        for a, b in zip(idx.search_rerank(queries, 10, 300), want):
+            np.testing.assert_array_equal(a, b)
+        for buf in (dq, d_ids, d_d, d_c):
+            buf.free()
         idx.set_tuning(0, 0, 0)
         assert rec(idx.search_rerank(queries, 10, 64)[0]) == r  # the tables were left empty
         idx.close()

@@ -266,9 +266,11 @@ int pk_index_init_build(pk_index *h, const uint8_t *levels);
  * batches of up to max_batch nodes (batches grow with the graph; a node that
  * opens a new top level goes alone).  Build (:497-507) is init_build + one
  * insert of all ids.  flags: PK_BUILD_EXACT_ROWS or 0.  Synchronous.
- * A device failure returns PK_ERR_DEVICE with its flags in the message
- * (1 a visited table filled up, 4 a candidate heap filled up, 8 a node lock
- * stayed taken for seconds). */
+ * A batch whose construction searches fill their visited tables reruns
+ * those searches with larger tables, up to 2^20 slots.  A device failure
+ * past that returns PK_ERR_DEVICE with its flags in the message (1 a visited
+ * table filled up, 4 a candidate heap filled up, 8 a node lock stayed taken
+ * for seconds). */
 int pk_index_insert(pk_index *h, const int32_t *ext_ids, int64_t n, int max_batch, int flags);
 
 /* ---- precision form: low-precision traversal copy + exact FP32 re-rank ----

@@ -270,7 +270,8 @@ int pk_index_init_build(pk_index *h, const uint8_t *levels);
  * those searches with larger tables, up to 2^20 slots.  A device failure
  * past that returns PK_ERR_DEVICE with its flags in the message (1 a visited
  * table filled up, 4 a candidate heap filled up, 8 a node lock stayed taken
- * for seconds). */
+ * for seconds, 16 a row already held 511 ids, the longest row AddEdge
+ * handles). */
 int pk_index_insert(pk_index *h, const int32_t *ext_ids, int64_t n, int max_batch, int flags);
 
 /* ---- precision form: low-precision traversal copy + exact FP32 re-rank ----

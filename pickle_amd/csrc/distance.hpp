@@ -67,7 +67,17 @@ struct IntDist {
         }
     }
 
+    // the same chunks from a row staged in LDS
+    __device__ void load_row_lds(Row& r, const uint8_t* row, int g) const {
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            const int c = g + 8 * i;
+            r.c[i] = ld16_lds(row + 16 * (c < nchunks ? c : 0));
+        }
+    }
+
     // every lane of the group returns the distance
+    template <bool LDS = false>
     __device__ float finish(const Row& r, int g, const uint8_t* row, bool active) const {
         uint32_t ab = 0, bb = 0;
 #pragma unroll
@@ -147,19 +157,37 @@ struct FloatDist {
                 r.raw[i] = gbl(reinterpret_cast<const uint32_t*>(row))[e];
         }
     }
+    // the same element bits from a row staged in LDS
+    __device__ void load_row_lds(Row& r, const uint8_t* row, int g) const {
+#pragma unroll
+        for (int i = 0; i < NEL; i++) {
+            const int e = i < nmain ? 16 * i + g : g;
+            if (HALF)
+                r.raw[i] = lds_ptr(reinterpret_cast<const uint16_t*>(row))[e];
+            else
+                r.raw[i] = lds_ptr(reinterpret_cast<const uint32_t*>(row))[e];
+        }
+    }
     __device__ static __forceinline__ float elem(uint32_t bits) {
         if (HALF) return (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
         return __uint_as_float(bits);
     }
+    template <bool LDS>
+    __device__ static __forceinline__ float row_elem(const uint8_t* row, int i) {
+        if (LDS) return qelem<HALF>(row, i);
+        return felem<HALF>(row, i);
+    }
 
-    // sequential tail in the compiled reference's order (first lane only)
+    // sequential tail in the compiled reference's order (first lane only);
+    // the row is in device memory, or in LDS when LDS is set
+    template <bool LDS = false>
     __device__ float tail(const uint8_t* row) const {
         int base = nmain * 16, j = 0;
         float r = 0.0f;
         if (rem >= 8) {
             float a[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) a[u] = fterm<MET>(qelem<HALF>(qst, base + u), felem<HALF>(row, base + u), 0.0f);
+            for (int u = 0; u < 8; u++) a[u] = fterm<MET>(qelem<HALF>(qst, base + u), row_elem<LDS>(row, base + u), 0.0f);
 #pragma unroll
             for (int h = 4; h >= 1; h >>= 1)
 #pragma unroll
@@ -171,23 +199,24 @@ struct FloatDist {
             float a[4];
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                a[u] = fterm<MET>(qelem<HALF>(qst, base + j + u), felem<HALF>(row, base + j + u), u == 0 ? r : 0.0f);
+                a[u] = fterm<MET>(qelem<HALF>(qst, base + j + u), row_elem<LDS>(row, base + j + u), u == 0 ? r : 0.0f);
             a[0] = a[0] + a[2];
             a[1] = a[1] + a[3];
             r = a[0] + a[1];
             j += 4;
         }
-        for (; j < rem; j++) r = fterm<MET>(qelem<HALF>(qst, base + j), felem<HALF>(row, base + j), r);
+        for (; j < rem; j++) r = fterm<MET>(qelem<HALF>(qst, base + j), row_elem<LDS>(row, base + j), r);
         return r;
     }
 
+    template <bool LDS = false>
     __device__ float finish(const Row& r, int g, const uint8_t* row, bool active) const {
         float acc = 0.0f;
 #pragma unroll
         for (int i = 0; i < NEL; i++)
             if (i < nmain) acc = fterm<MET>(q[i], elem(r.raw[i]), acc);
         float s = tree_sum16(acc);
-        if (rem != 0 && g == 0 && active) s = s + tail(row);
+        if (rem != 0 && g == 0 && active) s = s + tail<LDS>(row);
         return MET == MET_IP ? -s : s;
     }
 };
@@ -233,7 +262,16 @@ struct HalfDist {
         }
     }
 
+    __device__ void load_row_lds(Row& r, const uint8_t* row, int g) const {
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            const int c = g + 16 * i;
+            r.c[i] = ld16_lds(row + 16 * (c < nchunks ? c : 0));
+        }
+    }
+
     // every lane of the group returns the distance
+    template <bool LDS = false>
     __device__ float finish(const Row& r, int g, const uint8_t* /*row*/, bool /*active*/) const {
         float acc = 0.0f;
 #pragma unroll

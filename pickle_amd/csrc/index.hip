@@ -141,7 +141,8 @@ int publish_layers(pk_index* h) {
     return PK_OK;
 }
 
-int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits) {
+// Per-node lists of a batch (R and Pruned per level).
+static int ensure_insert_lists(pk_index* h, int64_t nodes, int maxl, int topk) {
     InsertScratch& s = h->ins;
     if (nodes > s.cap_nodes || maxl > s.cap_maxl || topk > s.topk) {
         dfree(s.d_ext);
@@ -150,6 +151,7 @@ int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int wa
         dfree(s.r_ids);
         dfree(s.r_d);
         dfree(s.r_cnt);
+        s.cap_nodes = 0;
         int rc;
         size_t lists = (size_t)nodes * maxl * 2;
         if ((rc = dalloc(&s.d_ext, nodes)) || (rc = dalloc(&s.d_lvl, nodes)) ||
@@ -160,21 +162,48 @@ int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int wa
         s.cap_maxl = maxl;
         s.topk = topk;
     }
-    if (waves > s.waves || vbits > s.vbits) {
-        dfree(s.cglob);
-        dfree(s.vtab);
-        dfree(s.vlog);
-        int rc;
-        size_t slots = (size_t)waves << vbits;
-        if ((rc = dalloc(&s.cglob, (size_t)waves * h->cg_cap)) || (rc = dalloc(&s.vtab, slots)) ||
-            (rc = dalloc(&s.vlog, slots)))
-            return rc;
-        HIPCHK(hipMemsetAsync(s.vtab, 0, slots * sizeof(uint32_t), h->stream));
-        s.waves = waves;
-        s.vbits = vbits;
-    }
+    return PK_OK;
+}
+
+// Visited tables, their logs and the candidate-heap spill areas of the
+// construction searches.  The first call sizes them for `waves` waves at
+// 2^vbits slots; a later, larger table (searches that filled theirs run
+// again) runs on fewer waves in the same memory (insert_scratch_waves), so
+// only a table too large for even one wave grows the allocation.  The
+// candidate heap has room for 2^vbits entries per wave: it only holds
+// first-visited ids, so it cannot fill before the visited table does.
+static int ensure_insert_tables(pk_index* h, int waves, int vbits) {
+    InsertScratch& s = h->ins;
+    const size_t cg_stride = std::max<size_t>((size_t)h->cg_cap, (size_t)1 << vbits);
+    if (s.v_slots > 0 && ((size_t)1 << vbits) <= s.v_slots && cg_stride <= s.cg_entries) return PK_OK;
+    const size_t need_v = std::max(s.v_slots, (size_t)std::max(waves, 1) << vbits);
+    const size_t need_cg = std::max(s.cg_entries, (size_t)std::max(waves, 1) * cg_stride);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(s.cglob);
+    dfree(s.vtab);
+    dfree(s.vlog);
+    s.v_slots = s.cg_entries = 0;
+    int rc;
+    if ((rc = dalloc(&s.cglob, need_cg)) || (rc = dalloc(&s.vtab, need_v)) || (rc = dalloc(&s.vlog, need_v)))
+        return rc;
+    HIPCHK(hipMemsetAsync(s.vtab, 0, need_v * sizeof(uint32_t), h->stream));
+    s.v_slots = need_v;
+    s.cg_entries = need_cg;
+    return PK_OK;
+}
+
+int insert_scratch_waves(const pk_index* h, int vbits) {
+    const InsertScratch& s = h->ins;
+    const size_t cg_stride = std::max<size_t>((size_t)h->cg_cap, (size_t)1 << vbits);
+    return (int)std::min<size_t>(s.v_slots >> vbits, s.cg_entries / cg_stride);
+}
+
+int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits) {
+    InsertScratch& s = h->ins;
+    int rc;
+    if ((rc = ensure_insert_lists(h, nodes, maxl, topk))) return rc;
+    if ((rc = ensure_insert_tables(h, waves, vbits))) return rc;
     if (!s.counter) {
-        int rc;
         // err[0]: flags; err[1..4]: the first failing node (ext id, level,
         // ids in its visited table, hops) for the error message
         if ((rc = dalloc(&s.counter, 1)) || (rc = dalloc(&s.err, 8))) return rc;
@@ -187,8 +216,7 @@ int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, con
                         int maxl) {
     InsertScratch& s = h->ins;
     if (nb > s.cap_nodes || maxl > s.cap_maxl) {
-        int rc = ensure_insert_scratch(h, std::max(nb, s.cap_nodes), std::max(maxl, s.cap_maxl), s.topk, s.waves,
-                                       s.vbits);
+        int rc = ensure_insert_lists(h, std::max(nb, s.cap_nodes), std::max(maxl, s.cap_maxl), s.topk);
         if (rc) return rc;
     }
     HIPCHK(hipMemcpy(s.d_ext, ext, nb * 4, hipMemcpyHostToDevice));
@@ -342,6 +370,9 @@ int pk_index_load_layer(pk_index* h, int level, int64_t n, int D, int64_t adj_le
     if (n < 0 || n > h->capacity || D < 1 || adj_len < n * D) return fail(PK_ERR_ARG, "bad layer shape");
     if (!h->base) return fail(PK_ERR_STATE, "set the base rows before loading layers");
     Layer& L = h->layers[level];
+    // no search may run on the layer from here until pk_index_set_enter_level
+    // republishes a complete graph, whatever fails below
+    h->ready = false;
     free_layer(L);
     L.n = n;
     L.cap = n;
@@ -525,6 +556,10 @@ static int search_class(const pk_index* h) {
     return h->size_class;
 }
 
+// vbits_override > 0 sets the visited-table size for this launch only (the
+// rerun of queries that filled their tables); such a launch takes no more
+// waves than the scratch already allocated holds at that size, since the
+// persistent queue serves any number of queries with fewer waves.
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
@@ -577,6 +612,11 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     if (t.blocks_per_cu > 0) bpc = std::min(bpc, t.blocks_per_cu);
     int64_t want = std::max<int64_t>(1, (nq + (kSearchBlock / 64) - 1) / (kSearchBlock / 64));
     int grid = (int)std::min<int64_t>(want, (int64_t)h->num_cus * bpc);
+    if (vbits_override > 0) {
+        const int64_t cg_stride = std::max<int64_t>(h->cg_cap, (int64_t)1 << vbits);
+        const int64_t fit = std::min<int64_t>((int64_t)(c.v_slots >> vbits), (int64_t)c.cg_entries / cg_stride);
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, fit / (kSearchBlock / 64)));
+    }
     int waves = grid * (kSearchBlock / 64);
     int rc = ensure_scratch(h, c, waves, vbits);
     if (rc) return rc;
@@ -688,6 +728,9 @@ int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k
                        float* out_ms) {
     if (!h || !d_queries || nq < 1 || k < 1 || ef < k) return fail(PK_ERR_ARG, "bad arguments");
     if (h->graphs_alive) return fail(PK_ERR_STATE, "destroy the search graphs before tuning");
+    // serving loops plan their launches from the handle's tuning, which the
+    // tuner rewrites while it runs
+    if (h->servers_alive) return fail(PK_ERR_STATE, "destroy the serving loops before tuning");
     int rc;
     uint8_t* buf = nullptr;
     const size_t out_bytes = (size_t)nq * k * 8 + (size_t)nq * 4;
@@ -855,7 +898,7 @@ int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, co
     // does not change results.  Not while captured search graphs pin the
     // scratch.
     constexpr int kVisitedFull = 1;
-    int vb = h->tuning.vbits > 0 ? h->tuning.vbits : default_vbits(ef_max, h->capacity);
+    int vb = h->sc.last_vbits;
     while (errflag == kVisitedFull && vb < 20 && h->graphs_alive == 0) {
         vb++;
         std::vector<int64_t> bad;
@@ -874,11 +917,8 @@ int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, co
         HIPCHK(hipMemcpyAsync(dq, q2.data(), q2.size(), hipMemcpyHostToDevice, h->stream));
         if (k) HIPCHK(hipMemcpyAsync(dk, k2.data(), (size_t)nb * 4, hipMemcpyHostToDevice, h->stream));
         if (ef) HIPCHK(hipMemcpyAsync(def, e2.data(), (size_t)nb * 4, hipMemcpyHostToDevice, h->stream));
-        const SearchTuning saved = h->tuning;
-        h->tuning.vbits = vb;
-        rc = pk_search_device_ex(h, dq, nb, (const int32_t*)dk, (const int32_t*)def, k_all, ef_all, ef_max, k_stride,
-                                 (int32_t*)dids, (float*)dd, (int32_t*)dc, (pk_query_counters*)dctr);
-        h->tuning = saved;
+        rc = search_on(h, h->sc, dq, nb, (const int32_t*)dk, (const int32_t*)def, k_all, ef_all, ef_max, k_stride,
+                       (int32_t*)dids, (float*)dd, (int32_t*)dc, (pk_query_counters*)dctr, vb);
         if (rc) return rc;
         std::vector<int32_t> ids2((size_t)nb * k_stride), cnt2(nb);
         std::vector<float> d2((size_t)nb * k_stride);
@@ -980,6 +1020,7 @@ int pk_timer_end(pk_index* h, float* ms) {
 
 int pk_set_tuning(pk_index* h, int ccap, int vbits, int blocks_per_cu) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (h->servers_alive) return fail(PK_ERR_STATE, "destroy the serving loops before changing the tuning");
     if (ccap < 0 || ccap > 16384 || (vbits != 0 && (vbits < 8 || vbits > 20)) || blocks_per_cu < 0)
         return fail(PK_ERR_ARG, "bad tuning (ccap <= 16384, vbits 0 or 8..20)");
     h->tuning.ccap = ccap;

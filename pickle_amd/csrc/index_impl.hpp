@@ -43,8 +43,9 @@ struct InsertScratch {
     uint64_t* cglob = nullptr;
     uint32_t* vtab = nullptr;
     int32_t* vlog = nullptr;
-    int waves = 0;
-    int vbits = 0;
+    size_t v_slots = 0;     // visited-table / log slots (any waves x 2^vbits layout that fits)
+    size_t cg_entries = 0;  // candidate-heap spill entries
+    int vbits = 0;          // visited-table size the last batch needed
     int32_t* counter = nullptr;
     int32_t* err = nullptr;
 };
@@ -86,6 +87,9 @@ namespace pk {
 void free_layer_public(Layer& L);
 int publish_layers(pk_index* h);
 int ensure_insert_scratch(pk_index* h, int64_t nodes, int maxl, int topk, int waves, int vbits);
+// waves whose visited tables of 2^vbits slots (and candidate heaps) fit the
+// insertion scratch
+int insert_scratch_waves(const pk_index* h, int vbits);
 int upload_insert_batch(pk_index* h, const int32_t* ext, const int32_t* lvl, const int32_t* in, int64_t nb,
                         int maxl);
 int gather_layer_vectors(pk_index* h, int level);

@@ -78,6 +78,8 @@ struct InsertArgs {
     int32_t cg_cap, vg_bits;
     int32_t wcap, ccap, lds_per_wave;
     int32_t exact_rows;
+    int32_t row_cache;       // AddEdge: rows short enough to stage whole in LDS (< row_cache ids)
+    int32_t row_cache_long;  // AddEdge on longer rows: kept-set vectors held in LDS
     int64_t rows_after[PK_MAX_LEVELS];  // layer sizes once the batch is in
     int32_t* qcounter;
     int32_t* err;
@@ -316,17 +318,74 @@ __device__ void lock_release(int32_t* lk, int lane) {
     if (lane == 0) atomicExch(lk, 0);
 }
 
+// The suffix rb[offset, dg) in ascending distance order, written to S as
+// (distance, slot t or id) entries, when that order is unique: both
+// segments [offset, h) and [h, dg) strictly increasing and no distance
+// shared between them -- then a min-heap of the entries pops exactly this
+// order.  Each entry's rank is its index in its segment plus a binary
+// search in the other.  Returns false (S unusable) otherwise.
+__device__ bool sorted_suffix(const float* d, const int* id, int offset, int h, int dg, bool by_slot, uint64_t* S,
+                              int lane) {
+    bool bad = false;
+    for (int t = offset + lane; t < dg; t += kWave) {
+        const float x = d[t];
+        const bool in1 = t < h;
+        const int own = in1 ? offset : h;
+        const int lo = in1 ? h : offset, hi = in1 ? dg : h;  // the other segment
+        if (t > own && !(d[t - 1] < x)) bad = true;
+        int first = lo, len = hi - lo;
+        while (len > 0) {
+            const int half = len >> 1;
+            const int mid = first + half;
+            if (d[mid] < x) {
+                first = mid + 1;
+                len = len - half - 1;
+            } else {
+                len = half;
+            }
+        }
+        if (first < hi && !(d[first] > x)) bad = true;  // a tie (or NaN)
+        S[(t - own) + (first - lo)] = pack_entry(x, by_slot ? t : id[t]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return ballot(bad) == 0;
+}
+
+// Is some member of the kept set nb_id[0, n) strictly closer than thr to
+// the row loaded as `dv`?  Members with an LDS slot (nb_slot, always a
+// prefix of the set) are read from the row cache, the others from device
+// memory; the scan stops at the first pass that finds one.
+template <class DF>
+__device__ __forceinline__ bool kept_closer(const DF& dv, const uint8_t* cache, const uint8_t* vec, int vstride,
+                                            const int* nb_id, const int* nb_slot, int in_lds, int n, float thr,
+                                            int lane) {
+    const int a = n < in_lds ? n : in_lds;
+    if (a > 0 && any_closer<DF, true>(dv, cache, vstride, nb_slot, a, thr, lane)) return true;
+    return n > a && any_closer<DF, false>(dv, vec, vstride, nb_id + a, n - a, thr, lane);
+}
+
+// AddEdge(vid, nid, distance) under vid's lock (hnsw_layer.hpp:360-465).
+// Vectors come from an LDS row cache wherever they fit:
+//  * short rows (dg + 1 <= A.row_cache): the whole row and the new node are
+//    staged once (row entry j in slot j, the new node in slot dg), so the
+//    occlusion test and the heuristic re-run read only LDS; the re-run's
+//    heaps then hold (distance, slot): their order depends on distances only;
+//  * long rows: the kept set's vectors are cached while they fit
+//    (A.row_cache_long slots, the rest gathered per step), and the next
+//    entry of the re-run -- the heap's next pop, which does not depend on
+//    the keep decisions -- is loaded while the current one is compared.
+// Every distance keeps the reference's summation order.
 template <class DF, bool IS_INT>
 __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_rows, int vid, int nid, float distance,
-                         int lane,
-                         int g, int qdim, int* rb_id, float* rb_d, int* nb_id, float* nb_d, uint64_t* h1,
-                         uint64_t* h2, float* cd, uint8_t* qst, int& fail) {
+                         int lane, int g, int qdim, int* rb_id, float* rb_d, int* nb_id, float* nb_d, int* nb_slot,
+                         float* cd, uint8_t* region, int& fail) {
     const bool leader = lane == 0;
     if (!lock_acquire(&B.lock[vid], lane)) {
         fail = kFailLockTimeout;
         return;
     }
     const size_t rowbase = (size_t)vid * B.D;
+    const int vs = A.vstride;
     // a row that grew into the end of the layer's allocation holds what fit
     // (the reference's flat array has no room there either)
     int dg = uni(B.deg[vid]);
@@ -334,7 +393,7 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
     int h = uni((int)B.heur[vid]);
     if (h > dg) h = dg;
     if (dg > kRowMax - 1) {
-        fail = 1;
+        fail = kFailRowLimit;
         lock_release(&B.lock[vid], lane);
         return;
     }
@@ -356,17 +415,57 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
     }
     const int offset = lower_bound_lds(rb_d, h, distance);
 
+    // region layout.  short rows: row_cache rows, then two heaps of
+    // row_cache entries.  long rows: kc kept-set rows, two staging rows
+    // (slots kc, kc + 1), then two heaps of kRowMax entries.
+    constexpr int kStage = 2;
+    const bool cached = dg + 1 <= A.row_cache;
+    uint8_t* cache = region;
+    const int kc = cached ? A.row_cache : A.row_cache_long;
+    const int hcap = cached ? A.row_cache : kRowMax;
+    uint64_t* h1 = reinterpret_cast<uint64_t*>(region + (size_t)(cached ? A.row_cache : kc + kStage) * vs);
+    uint64_t* h2 = h1 + hcap;
+
+    // the kept set of a re-run starts as the prefix plus the new node; in
+    // the long-row layout its entries past kc have no LDS slot (-1)
+    const int nid_slot = cached ? dg : (offset < kc ? offset : kc + 1);
+    for (int j = lane; j < offset; j += kWave) {
+        nb_id[j] = rb_id[j];
+        nb_d[j] = rb_d[j];
+        nb_slot[j] = j < kc ? j : -1;
+    }
+    if (leader) {
+        nb_id[offset] = nid;
+        nb_d[offset] = distance;
+        nb_slot[offset] = nid_slot;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // the prefix rows that have slots, and the new node, in one round trip
+    const int staged = offset < kc ? offset : kc;
+    {
+        // ids and slots of what is staged (nb_id / nb_slot [0, staged), then
+        // the new node), listed in the heaps' space before the heaps exist
+        int* sid = reinterpret_cast<int*>(h1);
+        int* sslot = sid + hcap;
+        for (int j = lane; j < staged; j += kWave) {
+            sid[j] = nb_id[j];
+            sslot[j] = nb_slot[j];
+        }
+        if (leader) {
+            sid[staged] = nid;
+            sslot[staged] = nid_slot;
+        }
+        __builtin_amdgcn_wave_barrier();
+        stage_rows(cache, B.vec, vs, sid, sslot, 0, staged + 1, lane);
+    }
+    const int in_lds0 = staged;  // prefix entries with LDS slots
+
     // is the new node occluded by a closer heuristic neighbour?
     bool insert_n = true;
     if (offset > 0) {
-        stage_row(qst, B.vec + (size_t)nid * A.vstride, A.vstride, A.vstride, lane);
         DF dn;
-        dn.load_query(qst, g, qdim);
-        batch_distances(dn, B.vec, A.vstride, rb_id, cd, offset, lane);
-        __builtin_amdgcn_wave_barrier();
-        bool occl = false;
-        for (int j = lane; j < offset; j += kWave) occl |= distance > cd[j];
-        insert_n = ballot(occl) == 0;
+        dn.load_query(cache + (size_t)nid_slot * vs, g, qdim);
+        insert_n = !kept_closer(dn, cache, B.vec, vs, nb_id, nb_slot, in_lds0, offset, distance, lane);
     }
 
     if (!insert_n) {
@@ -396,51 +495,105 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
         // slot D (hnsw_layer.hpp:420, 441-447); rows capped at D drop the
         // edge, and so do reference rows at their limit of kRowMax - 1 ids
     } else {
-        // re-run the heuristic on the entries from offset on (:409-463)
-        WaveHeap<false, false> W{h1, h1, kRowMax, 0, 0};
-        WaveHeap<false, false> Wd{h2, h2, kRowMax, 0, 0};
-        for (int t = offset; t < dg; t++) W.push_par(pack_entry(unif(rb_d[t]), uni(rb_id[t])), lane, leader);
-        for (int j = lane; j < offset; j += kWave) {
-            nb_id[j] = rb_id[j];
-            nb_d[j] = rb_d[j];
-        }
-        if (leader) {
-            nb_id[offset] = nid;
-            nb_d[offset] = distance;
-        }
-        __builtin_amdgcn_wave_barrier();
+        // re-run the heuristic on the entries from offset on (:409-463).
+        // The reference pops them from a min-heap W; when their distances are
+        // all distinct that order is the sorted one, built here by a
+        // parallel merge of the two sorted segments (`listed`), and the
+        // rejected entries then leave Wd in the order they entered it.  With
+        // ties the heaps run as the reference's do (their layout orders
+        // equal distances).
+        if (cached && dg > offset) stage_rows(cache, B.vec, vs, rb_id + offset, nullptr, offset, dg - offset, lane);
+        const int n_s = dg - offset;
+        uint64_t* S = h1;   // listed: the suffix in pop order
+        uint64_t* Sd = h2;  // listed: the rejected entries in order
+        const bool listed = sorted_suffix(rb_d, rb_id, offset, h, dg, cached, S, lane);
+        WaveHeap<false, false> W{h1, h1, hcap, 0, 0};
+        WaveHeap<false, false> Wd{h2, h2, hcap, 0, 0};
+        if (!listed)
+            for (int t = offset; t < dg; t++) W.push_par(pack_entry(unif(rb_d[t]), cached ? t : uni(rb_id[t])), lane, leader);
+        int it = 0, nrej = 0;
         int cur_h = offset + 1;
-        while (W.len > 0) {
-            const uint64_t v1 = W.top();
-            W.pop_par(lane, leader);
-            stage_row(qst, B.vec + (size_t)entry_id(v1) * A.vstride, A.vstride, A.vstride, lane);
+        int in_lds = cached ? cur_h : (nid_slot < kc ? offset + 1 : in_lds0);  // kept entries with LDS slots
+        // long rows: v1's row in staging slot `cur`, the next one loading into the other
+        int cur = kc + 1;
+        const int cpr = vs >> 4;
+        if (!cached && n_s > 0)
+            stage_one_row(cache + (size_t)cur * vs, B.vec + (size_t)entry_id(listed ? uni64(S[0]) : W.top()) * vs, vs,
+                          lane);
+        while (listed ? it < n_s : W.len > 0) {
+            uint64_t v1;
+            if (listed) {
+                v1 = uni64(S[it]);
+                it++;
+            } else {
+                v1 = W.top();
+                W.pop_par(lane, leader);
+            }
+            const bool more = listed ? it < n_s : W.len > 0;
+            const int next_key = more ? entry_id(listed ? uni64(S[it]) : W.top()) : 0;
+            const int v1_slot = cached ? entry_id(v1) : cur;
+            const int v1_id = cached ? uni(rb_id[v1_slot]) : entry_id(v1);
+            // long rows: the next entry's row is loaded while this one is compared
+            uint4 pf0 = make_uint4(0, 0, 0, 0), pf1 = make_uint4(0, 0, 0, 0);
+            const bool prefetch = !cached && more && cpr <= 2 * kWave;
+            if (prefetch) {
+                const uint8_t* nrow = B.vec + (size_t)next_key * vs;
+                if (lane < cpr) pf0 = ld16_global(nrow + 16 * lane);
+                if (lane + kWave < cpr) pf1 = ld16_global(nrow + 16 * (lane + kWave));
+            }
             DF dv;
-            dv.load_query(qst, g, qdim);
-            batch_distances(dv, B.vec, A.vstride, nb_id, cd, cur_h, lane);
-            __builtin_amdgcn_wave_barrier();
-            bool occl = false;
-            for (int j = lane; j < cur_h; j += kWave) occl |= cd[j] < entry_dist(v1);
-            const bool keep = ballot(occl) == 0 && (A.exact_rows || cur_h < B.D) && cur_h < kRowMax - 1;
+            dv.load_query(cache + (size_t)v1_slot * vs, g, qdim);
+            const bool occl = kept_closer(dv, cache, B.vec, vs, nb_id, nb_slot, in_lds, cur_h, entry_dist(v1), lane);
+            const bool keep = !occl && (A.exact_rows || cur_h < B.D) && cur_h < kRowMax - 1;
             if (keep) {
+                int slot = v1_slot;
+                if (!cached) {
+                    // the kept row joins the LDS part of the set while it fits
+                    slot = -1;
+                    if (in_lds == cur_h && cur_h < kc) {
+                        slot = cur_h;
+                        for (int c = lane; c < cpr; c += kWave)
+                            *reinterpret_cast<uint4*>(cache + (size_t)slot * vs + 16 * c) =
+                                ld16_lds(cache + (size_t)v1_slot * vs + 16 * c);
+                        in_lds++;
+                    }
+                } else {
+                    in_lds++;
+                }
                 if (leader) {
-                    nb_id[cur_h] = entry_id(v1);
+                    nb_id[cur_h] = v1_id;
                     nb_d[cur_h] = entry_dist(v1);
+                    nb_slot[cur_h] = slot;
                 }
                 cur_h++;
+            } else if (listed) {
+                if (leader) Sd[nrej] = v1;
+                nrej++;
             } else {
                 Wd.push_par(v1, lane, leader);
+            }
+            if (!cached && more) {
+                // the next entry's row into the other staging slot
+                cur = cur == kc ? kc + 1 : kc;
+                if (prefetch) {
+                    if (lane < cpr) *reinterpret_cast<uint4*>(cache + (size_t)cur * vs + 16 * lane) = pf0;
+                    if (lane + kWave < cpr) *reinterpret_cast<uint4*>(cache + (size_t)cur * vs + 16 * (lane + kWave)) = pf1;
+                } else {
+                    stage_one_row(cache + (size_t)cur * vs, B.vec + (size_t)next_key * vs, vs, lane);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         }
         int cur_deg = cur_h;
-        while (cur_deg < B.D && Wd.len > 0) {
-            const uint64_t e = Wd.top();
+        for (int r = 0; cur_deg < B.D && (listed ? r < nrej : Wd.len > 0); r++) {
+            const uint64_t e = listed ? uni64(Sd[r]) : Wd.top();
+            const int e_id = cached ? uni(rb_id[entry_id(e)]) : entry_id(e);
             if (leader) {
-                nb_id[cur_deg] = entry_id(e);
+                nb_id[cur_deg] = e_id;
                 nb_d[cur_deg] = entry_dist(e);
             }
             cur_deg++;
-            Wd.pop_par(lane, leader);
+            if (!listed) Wd.pop_par(lane, leader);
         }
         __builtin_amdgcn_wave_barrier();
         // flat row offsets, as the reference's spans: with exact rows a row
@@ -466,10 +619,10 @@ __global__ __launch_bounds__(kInsBlock) void k_insert_edges(InsertArgs A) {
     float* rb_d = reinterpret_cast<float*>(rb_id + kRowMax);
     int* nb_id = reinterpret_cast<int*>(rb_d + kRowMax);
     float* nb_d = reinterpret_cast<float*>(nb_id + kRowMax);
-    uint64_t* h1 = reinterpret_cast<uint64_t*>(nb_d + kRowMax);
-    uint64_t* h2 = h1 + kRowMax;
-    float* cd = reinterpret_cast<float*>(h2 + kRowMax);
-    uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kRowMax);
+    float* cd = reinterpret_cast<float*>(nb_d + kRowMax);
+    int* nb_slot = reinterpret_cast<int*>(cd + kRowMax);
+    // row cache and heaps (see add_edge)
+    uint8_t* region = reinterpret_cast<uint8_t*>(nb_slot + kRowMax);
     const int g = lane % DF::LPC;
     const int qdim = IS_INT ? (A.vstride >> 4) : A.dim;
     for (;;) {
@@ -492,7 +645,7 @@ __global__ __launch_bounds__(kInsBlock) void k_insert_edges(InsertArgs A) {
             for (int t = 0; t < nR + nP && !fail; t++) {
                 const int src = t < nR ? t : A.topk + (t - nR);
                 add_edge<DF, IS_INT>(A, B, n_rows, uni(sid[src]), vid, unif(sd[src]), lane, g, qdim, rb_id, rb_d,
-                                     nb_id, nb_d, h1, h2, cd, qst, fail);
+                                     nb_id, nb_d, nb_slot, cd, region, fail);
             }
         }
         if (fail && lane == 0) atomicOr(A.err, fail);
@@ -689,8 +842,30 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
     const int ccap = 1024;
     size_t lds_a = (size_t)wcap * 8 * 3 + (size_t)ccap * 8 + kTopKMax * 8 + 64 * 4 + kTopKMax * 4 + 2 * h->vstride;
     lds_a = (lds_a + 15) / 16 * 16;
-    size_t lds_c = (size_t)kRowMax * (4 * 4 + 8 * 2 + 4) + h->vstride;
-    lds_c = (lds_c + 15) / 16 * 16;
+    // AddEdge: six row lists and the staged new node, then a region that
+    // holds the LDS row cache with two heaps of its size, or two heaps of
+    // kRowMax entries for rows too long for the cache.  The cache takes what
+    // keeps the workgroup within 52 KB, three resident per CU (76 rows of
+    // float32 d = 128: a row of 2M = 64 ids plus the new node fits).
+    //   short rows: row_cache rows + two heaps of row_cache entries
+    //   long rows:  row_cache_long kept-set rows + two staging rows + two
+    //               heaps of kRowMax entries, in the same space
+    const size_t vs = (size_t)h->vstride;
+    const size_t lds_c0 = (size_t)kRowMax * 4 * 6;
+    const size_t long_heaps = (size_t)kRowMax * 8 * 2;
+    constexpr size_t kEdgeLds = 52 * 1024;
+    int row_cache = lds_c0 + 2 * (vs + 16) > kEdgeLds
+                        ? 0
+                        : (int)std::min<size_t>(kRowMax, (kEdgeLds - lds_c0) / (vs + 16));
+    if (row_cache < 2) row_cache = 0;
+    const size_t short_region = (size_t)row_cache * (vs + 16);
+    const size_t stage_rows_long = 2;
+    const int row_cache_long =
+        short_region > long_heaps + stage_rows_long * vs
+            ? (int)std::min<size_t>(kRowMax / 2, (short_region - long_heaps) / vs - stage_rows_long)
+            : 0;
+    const size_t region = std::max(short_region, long_heaps + (row_cache_long + stage_rows_long) * vs);
+    const size_t lds_c = (lds_c0 + region + 15) / 16 * 16;
     int grid_cap = h->num_cus * 8;
     // scratch for the largest batch
     const int maxl_all = [&] {
@@ -777,16 +952,18 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
         a.cglob = h->ins.cglob;
         a.vtab = h->ins.vtab;
         a.vlog = h->ins.vlog;
-        a.cg_cap = h->cg_cap;
+        a.cg_cap = std::max(h->cg_cap, 1 << vbits);
         a.vg_bits = vbits;
         a.wcap = wcap;
         a.ccap = ccap;
         a.lds_per_wave = (int)lds_a;
         a.exact_rows = (flags & PK_BUILD_EXACT_ROWS) != 0;
+        a.row_cache = row_cache;
+        a.row_cache_long = row_cache_long;
         for (int l = 0; l < PK_MAX_LEVELS; l++) a.rows_after[l] = h->layers[l].n;
         a.qcounter = h->ins.counter;
         a.err = h->ins.err;
-        const int grid = (int)std::min<int64_t>(nb, grid_cap);
+        int grid = (int)std::min<int64_t>({nb, (int64_t)grid_cap, (int64_t)pk::insert_scratch_waves(h, vbits)});
         for (;;) {
             ICHK(launch_insert(a, 0, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
             int32_t err = 0;
@@ -795,10 +972,14 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
             if (err != kFailVisitedFull || vbits >= 20)
                 return set_error(PK_ERR_DEVICE, "insertion search failed on the device" + diag_text());
             vbits++;
+            h->ins.vbits = vbits;
+            // the larger tables run on fewer waves in the memory already held
             if ((rc = pk::ensure_insert_scratch(h, max_batch, maxl_all, topk, grid_cap, vbits))) return rc;
+            grid = (int)std::min<int64_t>({nb, (int64_t)grid_cap, (int64_t)pk::insert_scratch_waves(h, vbits)});
             a.vtab = h->ins.vtab;
             a.vlog = h->ins.vlog;
             a.cglob = h->ins.cglob;
+            a.cg_cap = std::max(h->cg_cap, 1 << vbits);
             a.vg_bits = vbits;
             ICHK(hipMemsetAsync(h->ins.err, 0, 32, h->stream));
         }
@@ -807,7 +988,9 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
             int32_t err = 0;
             if ((rc = device_flags(&err))) return rc;
             if (err)
-                return set_error(PK_ERR_DEVICE, "AddEdge failed on the device (flags " + std::to_string(err) + ")");
+                return set_error(PK_ERR_DEVICE, "AddEdge failed on the device (flags " + std::to_string(err) +
+                                                    ": 8 = a node lock was held for seconds, 16 = a row reached " +
+                                                    std::to_string(kRowMax - 1) + " ids)");
         }
         // the new sizes become visible, then SetEntLevel (hnsw.hpp:146-151):
         // a search launched from another thread that reads the new enter

@@ -301,6 +301,9 @@ int pk_index_copy_graph(pk_index* dst, pk_index* src) {
         return set_error(PK_ERR_STATE, "destroy the destination's search graphs and serving loops first");
     PCHK(hipStreamSynchronize(src->stream));
     PCHK(hipStreamSynchronize(dst->stream));
+    // the destination's layers are freed below: no search may use them until
+    // pk_index_set_enter_level publishes the copy, whatever fails meanwhile
+    dst->ready = false;
     int rc;
     for (int l = 0; l < PK_MAX_LEVELS; l++) {
         Layer& S = src->layers[l];
@@ -334,9 +337,14 @@ int pk_index_copy_graph(pk_index* dst, pk_index* src) {
     return pk_index_set_enter_level(dst, src->enter_level);
 }
 
-int pk_search_rerank_device(pk_index* h, const float* d_queries, int64_t nq, int32_t k, int32_t ef,
-                            int32_t k_stride, int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count,
-                            pk_query_counters* d_counters) {
+}  // extern "C"
+
+// pk_search_rerank_device with the traversal's visited-table size set for
+// this call only (vbits_override > 0: the rerun of queries that filled
+// their tables, on no more waves than the allocated scratch holds)
+static int rerank_device(pk_index* h, const float* d_queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,
+                         int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
+                         int vbits_override) {
     if (!h) return set_error(PK_ERR_ARG, "null handle");
     int rc = check_precision_form(h);
     if (rc) return rc;
@@ -352,8 +360,8 @@ int pk_search_rerank_device(pk_index* h, const float* d_queries, int64_t nq, int
         return rc;
     // queries -> traversal type (rows of qbytes, as the search reads them)
     if ((rc = quantize_rows(h, d_queries, h->dim, nq, static_cast<uint8_t*>(qc), h->qbytes))) return rc;
-    rc = pk_search_device_ex(h, qc, nq, nullptr, nullptr, ef, ef, ef, ef, static_cast<int32_t*>(cids),
-                             static_cast<float*>(cd), static_cast<int32_t*>(cc), d_counters);
+    rc = search_on(h, h->sc, qc, nq, nullptr, nullptr, ef, ef, ef, ef, static_cast<int32_t*>(cids),
+                   static_cast<float*>(cd), static_cast<int32_t*>(cc), d_counters, vbits_override);
     if (rc) return rc;
     RerankArgs a{};
     a.queries = d_queries;
@@ -378,6 +386,14 @@ int pk_search_rerank_device(pk_index* h, const float* d_queries, int64_t nq, int
     const int grid = (int)std::min<int64_t>((nq + 1) / 2, (int64_t)h->num_cus * 8);
     PCHK(launch_rerank(a, h->metric, nel, grid, lds, h->stream));
     return PK_OK;
+}
+
+extern "C" {
+
+int pk_search_rerank_device(pk_index* h, const float* d_queries, int64_t nq, int32_t k, int32_t ef,
+                            int32_t k_stride, int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count,
+                            pk_query_counters* d_counters) {
+    return rerank_device(h, d_queries, nq, k, ef, k_stride, d_out_ids, d_out_dists, d_out_count, d_counters, 0);
 }
 
 int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,
@@ -435,12 +451,9 @@ int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, i
         for (int64_t j = 0; j < nb; j++) std::memcpy(&q2[(size_t)j * dim], queries + bad[j] * dim, (size_t)dim * 4);
         if (hipMemcpy(dq, q2.data(), q2.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return done(set_error(PK_ERR_HIP, "query upload failed"));
-        const int saved = h->tuning.vbits;
-        h->tuning.vbits = vb;
-        rc = pk_search_rerank_device(h, static_cast<const float*>(dq), nb, k, ef, k_stride, static_cast<int32_t*>(di),
-                                     static_cast<float*>(dd), static_cast<int32_t*>(dc),
-                                     static_cast<pk_query_counters*>(dctr));
-        h->tuning.vbits = saved;
+        rc = rerank_device(h, static_cast<const float*>(dq), nb, k, ef, k_stride, static_cast<int32_t*>(di),
+                           static_cast<float*>(dd), static_cast<int32_t*>(dc), static_cast<pk_query_counters*>(dctr),
+                           vb);
         if (rc) return done(rc);
         std::vector<int32_t> ids2((size_t)nb * k_stride), cnt2((size_t)nb);
         std::vector<float> d2((size_t)nb * k_stride);

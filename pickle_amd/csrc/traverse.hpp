@@ -77,6 +77,108 @@ __device__ void batch_distances(const DF& df, const uint8_t* vec, int vstride, c
     }
 }
 
+// batch_distances for rows staged in LDS: row ci is rows + slot[ci] * vstride
+template <class DF>
+__device__ void batch_distances_lds(const DF& df, const uint8_t* rows, int vstride, const int* slot, float* cd,
+                                    int ncand, int lane) {
+    constexpr int LPC = DF::LPC;
+    constexpr int G = kWave / LPC;
+    constexpr int PB = PassesOf<DF>::value;
+    const int grp = lane / LPC, g = lane % LPC;
+    for (int base = 0; base < ncand; base += G * PB) {
+        typename DF::Row r[PB];
+        const uint8_t* rp[PB];
+        bool act[PB];
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            const int ci = base + p * G + grp;
+            act[p] = ci < ncand;
+            rp[p] = rows + (size_t)slot[act[p] ? ci : 0] * (size_t)vstride;
+            df.load_row_lds(r[p], rp[p], g);
+        }
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            float d = df.template finish<true>(r[p], g, rp[p], act[p]);
+            int ci = base + p * G + grp;
+            if (act[p] && g == 0) cd[ci] = d;
+        }
+    }
+}
+
+// Copies rows ids[0, n) of vec (vstride bytes each, ids valid rows) to the
+// LDS rows dst + slot * vstride, slot = slots[j] (or slot_base + j when
+// slots is null), with up to 16 16-byte loads in flight per lane.
+__device__ __forceinline__ void stage_rows(uint8_t* dst, const uint8_t* vec, int vstride, const int* ids,
+                                           const int* slots, int slot_base, int n, int lane) {
+    const int cpr = vstride >> 4;  // chunks per row
+    const int total = n * cpr;
+    constexpr int U = 16;
+    for (int c0 = 0; c0 < total; c0 += kWave * U) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + u * kWave + lane;
+            const int r = c < total ? c / cpr : 0;
+            const int k = c < total ? c - r * cpr : 0;
+            v[u] = ld16_global(vec + (size_t)ids[r] * (size_t)vstride + 16 * k);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + u * kWave + lane;
+            if (c < total) {
+                const int r = c / cpr;
+                const int slot = slots ? slots[r] : slot_base + r;
+                *reinterpret_cast<uint4*>(dst + (size_t)slot * vstride + 16 * (c - r * cpr)) = v[u];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// True when some row ci < ncand is strictly closer than thr to the query of
+// df: rows + slot[ci] * vstride in LDS (FROM_LDS) or vec + id[ci] * vstride
+// in device memory.  Rows are compared a pass at a time (G * PB rows) and
+// the scan stops at the pass holding the first such row, as the
+// reference's loops break at their first occluder.
+template <class DF, bool FROM_LDS>
+__device__ bool any_closer(const DF& df, const uint8_t* rows, int vstride, const int* list, int ncand, float thr,
+                           int lane) {
+    constexpr int LPC = DF::LPC;
+    constexpr int G = kWave / LPC;
+    constexpr int PB = PassesOf<DF>::value;
+    const int grp = lane / LPC, g = lane % LPC;
+    for (int base = 0; base < ncand; base += G * PB) {
+        typename DF::Row r[PB];
+        const uint8_t* rp[PB];
+        bool act[PB];
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            const int ci = base + p * G + grp;
+            act[p] = ci < ncand;
+            rp[p] = rows + (size_t)list[act[p] ? ci : 0] * (size_t)vstride;
+            if (FROM_LDS)
+                df.load_row_lds(r[p], rp[p], g);
+            else
+                df.load_row(r[p], rp[p], g, act[p]);
+        }
+        bool closer = false;
+#pragma unroll
+        for (int p = 0; p < PB; p++) {
+            const float d = df.template finish<FROM_LDS>(r[p], g, rp[p], act[p]);
+            closer |= act[p] && g == 0 && d < thr;
+        }
+        if (ballot(closer)) return true;
+    }
+    return false;
+}
+
+// One row of vstride bytes (a multiple of 16) from device memory to LDS.
+__device__ __forceinline__ void stage_one_row(uint8_t* dst, const uint8_t* src, int vstride, int lane) {
+    for (int c = lane; c < (vstride >> 4); c += kWave)
+        *reinterpret_cast<uint4*>(dst + 16 * c) = ld16_global(src + 16 * c);
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ int clamp_degree(const LayerDev& L, int v, int deg) {
     int64_t room = L.adj_len - (int64_t)v * L.D;
     if (deg < 0) deg = 0;
@@ -141,6 +243,7 @@ constexpr int kFailVisitedFull = 1;
 constexpr int kFailArgs = 2;
 constexpr int kFailHeapFull = 4;
 constexpr int kFailLockTimeout = 8;  // insertion: a node lock was held for seconds
+constexpr int kFailRowLimit = 16;    // insertion: a row already holds the longest supported run of ids
 
 struct WorkCount {
     int hops = 0, neighbors = 0, dists = 0;

@@ -155,6 +155,24 @@ int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const in
                         int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
                         int32_t *d_out_count, pk_query_counters *d_counters);
 
+/* pk_search_device_ex without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels run with all of their device scratch (work counter, failure flags,
+ * visited tables and logs, candidate-heap spill) in one static device array
+ * of 128 MiB per device, held until the process ends; the call makes no
+ * hipMalloc, hipFree or other allocating call.  Launches take as many waves
+ * as their visited tables fit in the array (one workgroup needs 2 x 2^vbits
+ * x 16 B plus the heap spill; every supported size fits), so a large batch
+ * runs on fewer waves than pk_search_device_ex uses.  A query that fills its
+ * table ends with count -1, as in pk_search_device_ex; its flags are not
+ * reported through pk_device_error.  Calls on one device share the array:
+ * calls that overlap in time on one device race. */
+int pk_search_device_static(pk_index *h, const void *d_queries, int64_t nq, const int32_t *d_k,
+                            const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max,
+                            int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
+                            int32_t *d_out_count, pk_query_counters *d_counters);
+
 /* One pk_search_device_ex launch captured into a HIP graph (no counterpart
  * in the reference): the same buffers, shapes and per-query k / ef arrays
  * are replayed by pk_search_graph_launch on the index's stream, so a serving

@@ -27,6 +27,7 @@ EXPORTS = [
     "pk_last_error", "pk_version", "pk_index_create", "pk_index_destroy", "pk_index_set_base",
     "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
+    "pk_search_device_static",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
     "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
@@ -73,6 +74,7 @@ def lib():
             "pk_search": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device_ex": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_device_static": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_device_error": (C.c_int, [vp, vp]),
             "pk_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "pk_last_launch": (C.c_int, [vp, vp, vp]),

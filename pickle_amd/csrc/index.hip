@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -262,6 +263,7 @@ int pk_index_create(int dtype, int metric, int dim, int M, int efc, int64_t capa
     h->sc.stream = h->stream;
     int dev = 0;
     (void)hipGetDevice(&dev);
+    h->device = dev;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) h->num_cus = prop.multiProcessorCount;
     if (h->num_cus <= 0) h->num_cus = 256;
@@ -534,6 +536,7 @@ struct SearchPlan {
     SearchArgs a;
     int grid = 0;
     size_t lds = 0;
+    int vbits = 0;  // visited-table size of the launch (log2 slots per wave)
 };
 
 // validation, LDS plan and scratch (the part that may allocate)
@@ -563,7 +566,7 @@ static int search_class(const pk_index* h) {
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
-                       SearchPlan* P, int vbits_override = 0) {
+                       SearchPlan* P, int vbits_override = 0, bool bind_scratch = true) {
     if (!h) return fail(PK_ERR_ARG, "null handle");
     if (!h->ready) return fail(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
     if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return fail(PK_ERR_ARG, "bad buffers");
@@ -617,6 +620,14 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
         const int64_t fit = std::min<int64_t>((int64_t)(c.v_slots >> vbits), (int64_t)c.cg_entries / cg_stride);
         grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, fit / (kSearchBlock / 64)));
     }
+    a.vg_bits = vbits;
+    a.wcap = wcap;
+    a.ccap = ccap;
+    a.lds_per_wave = (int)per_wave;
+    P->grid = grid;
+    P->lds = lds;
+    P->vbits = vbits;
+    if (!bind_scratch) return PK_OK;  // the caller places the scratch
     int waves = grid * (kSearchBlock / 64);
     int rc = ensure_scratch(h, c, waves, vbits);
     if (rc) return rc;
@@ -625,14 +636,8 @@ static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t
     a.vtab = c.vtab;
     a.vlog = c.vlog;
     a.cg_cap = c.cg_cap;
-    a.vg_bits = vbits;
-    a.wcap = wcap;
-    a.ccap = ccap;
-    a.lds_per_wave = (int)per_wave;
     a.qcounter = c.qcounter;
     a.err = c.err;
-    P->grid = grid;
-    P->lds = lds;
     return PK_OK;
 }
 
@@ -817,6 +822,106 @@ int pk_autotune_search(pk_index* h, const void* d_queries, int64_t nq, int32_t k
         out_tuning[2] = best.blocks_per_cu;
     }
     if (out_ms) *out_ms = best_ms;
+    return PK_OK;
+}
+
+}  // extern "C"
+
+// ---- the search without allocation: scratch in a static device array ----
+//
+// pk_search_device_static runs the kernels of pk_search_device_ex with
+// every piece of device scratch -- the work counter, the failure flags, the
+// visited tables with their logs and the candidate-heap spill areas -- in
+// the file-scope array `scratch` (kStaticScratchBytes per device, held until
+// the process ends).  Host code never names the array as an address: the
+// device address comes from hipGetSymbolAddress, once per device with that
+// device current, and is kept per device.  The visited tables are zero
+// between searches (each search clears what it marked); the first call on a
+// device zeroes the whole array with a fill kernel.  Calls that overlap on
+// one device share the array and race.
+namespace {
+// fixed regions, so the visited tables occupy the same bytes whatever a
+// call's table size (they are zero between searches; the logs and the heap
+// spill are not): head (work counter, failure flags), tables, logs, spill
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+constexpr size_t kStaticHead = 256;
+constexpr size_t kStaticTables = (size_t)32 << 20;
+constexpr size_t kStaticLogs = (size_t)32 << 20;
+constexpr size_t kStaticSpill = kStaticScratchBytes - kStaticHead - kStaticTables - kStaticLogs;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+std::mutex g_static_mu;
+uint8_t* g_static_ptr[kMaxDevices] = {};
+bool g_static_zeroed[kMaxDevices] = {};
+
+__global__ void k_zero16(uint4* p, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+// the array's device address on h's device (looked up on first use there)
+int static_scratch(pk_index* h, uint8_t** out) {
+    if (h->device < 0 || h->device >= kMaxDevices) return fail(PK_ERR_STATE, "device %d out of range", h->device);
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_ptr[h->device]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != h->device) HIPCHK(hipSetDevice(h->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != h->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(PK_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_static_ptr[h->device] = static_cast<uint8_t*>(p);
+    }
+    if (!g_static_zeroed[h->device]) {
+        hipLaunchKernelGGL(k_zero16, dim3(1024), dim3(256), 0, h->stream,
+                           reinterpret_cast<uint4*>(g_static_ptr[h->device]), kStaticScratchBytes / 16);
+        HIPCHK(hipGetLastError());
+        g_static_zeroed[h->device] = true;
+    }
+    *out = g_static_ptr[h->device];
+    return PK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pk_search_device_static(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                            int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                            float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (nq == 0 && d_out_ids) return PK_OK;
+    SearchPlan P;
+    int rc = plan_search(h, h->sc, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids, d_out_dists,
+                         d_out_count, d_counters, &P, 0, /*bind_scratch=*/false);
+    if (rc) return rc;
+    if (nq == 0) return PK_OK;
+    // per wave: visited table and log (4 B per slot each), candidate-heap
+    // spill (8 B per entry, room for 2^vbits); the persistent queue serves
+    // any number of queries with the waves that fit
+    const size_t slots = (size_t)1 << P.vbits;
+    const size_t cg_stride = std::max<size_t>((size_t)h->cg_cap, slots);
+    const size_t waves_fit = std::min(kStaticTables / (slots * 4), kStaticSpill / (cg_stride * 8));
+    constexpr int kWavesPerGroup = kSearchBlock / 64;
+    if (waves_fit < (size_t)kWavesPerGroup)
+        return fail(PK_ERR_CAPACITY,
+                    "a visited table of 2^%d slots needs %zu bytes of tables and %zu of heap spill per workgroup; the "
+                    "static scratch holds %zu and %zu",
+                    P.vbits, slots * 4 * kWavesPerGroup, cg_stride * 8 * kWavesPerGroup, kStaticTables, kStaticSpill);
+    P.grid = (int)std::min<size_t>((size_t)P.grid, waves_fit / kWavesPerGroup);
+    uint8_t* base = nullptr;
+    if ((rc = static_scratch(h, &base))) return rc;
+    SearchArgs& a = P.a;
+    a.qcounter = reinterpret_cast<int32_t*>(base);
+    a.err = reinterpret_cast<int32_t*>(base) + 1;
+    a.vtab = reinterpret_cast<uint32_t*>(base + kStaticHead);
+    a.vlog = reinterpret_cast<int32_t*>(base + kStaticHead + kStaticTables);
+    a.cglob = reinterpret_cast<uint64_t*>(base + kStaticHead + kStaticTables + kStaticLogs);
+    a.cg_cap = (int)cg_stride;
+    HIPCHK(launch_search(a, h->dtype, h->metric, search_class(h), P.grid, P.lds, h->stream));
+    h->last_grid = P.grid;
+    h->last_lds = P.lds;
     return PK_OK;
 }
 

@@ -113,6 +113,7 @@ struct pk_index {
     int vstride = 0;  // bytes per stored row (16-byte multiple, zero padded)
     int size_class = 0;
     int num_cus = 0;
+    int device = 0;  // the device current when the handle was created
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint8_t* base = nullptr;

@@ -155,6 +155,29 @@ int pk_search_device_ex(pk_index *h, const void *d_queries, int64_t nq, const in
                         int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
                         int32_t *d_out_count, pk_query_counters *d_counters);
 
+/* Fast mode of the batched search (no counterpart in the reference; its
+ * caller is the same AnnSearch(k, ef, NDArray), include/hnsw.hpp:528-541).
+ * The reference's accept and stopping rules (hnsw.hpp:398-434) run on a
+ * sorted beam kept in registers: each round expands the `width` closest
+ * unexpanded entries together, new neighbours are merged with wave bitonic
+ * networks, and the visited filter lives in LDS (forgotten ids are only
+ * evaluated again, never returned twice).  Results are recall-equivalent,
+ * not bit-exact: ties are ordered by id, and width > 1 expands a few more
+ * nodes.  Device buffers, enqueued on the handle's stream; ef_max <= 512;
+ * no visited state in device memory.  Failures: count -1 and
+ * pk_device_error flags as pk_search_device_ex. */
+int pk_search_fast_device(pk_index *h, const void *d_queries, int64_t nq, const int32_t *d_k,
+                          const int32_t *d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max,
+                          int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
+                          int32_t *d_out_count, pk_query_counters *d_counters);
+
+/* Fast-mode tuning (0 = automatic for each): width 1..4 entries expanded
+ * per round (default 2), hash_bits 8..14 (log2 slots of each query's LDS
+ * visited filter, default 12), waves_per_group 1, 2 or 4 (queries per
+ * workgroup, default 2), blocks_per_cu (cap on resident workgroups).  No
+ * tuning changes a result except width. */
+int pk_set_fast_tuning(pk_index *h, int width, int hash_bits, int waves_per_group, int blocks_per_cu);
+
 /* pk_search_device_ex without allocation.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The same

@@ -27,7 +27,7 @@ EXPORTS = [
     "pk_last_error", "pk_version", "pk_index_create", "pk_index_destroy", "pk_index_set_base",
     "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
-    "pk_search_device_static",
+    "pk_search_device_static", "pk_search_fast_device", "pk_set_fast_tuning",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
     "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
@@ -75,6 +75,8 @@ def lib():
             "pk_search_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device_ex": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device_static": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_fast_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_set_fast_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "pk_device_error": (C.c_int, [vp, vp]),
             "pk_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "pk_last_launch": (C.c_int, [vp, vp, vp]),
@@ -380,6 +382,53 @@ class Index:
         check(lib().pk_search_device_ex(self.h, d_queries.ptr, nq, d_k.ptr if d_k else None, d_ef.ptr if d_ef else None,
                                         k, ef, ef_max if ef_max else ef, k_stride, d_ids.ptr, d_dists.ptr,
                                         d_count.ptr if d_count else None, d_ctr.ptr if d_ctr else None))
+
+    def search_fast_device(self, d_queries: DeviceBuffer, nq: int, k: int, ef: int, k_stride: int,
+                           d_ids: DeviceBuffer, d_dists: DeviceBuffer, d_count: DeviceBuffer | None = None,
+                           d_ctr: DeviceBuffer | None = None, d_k: DeviceBuffer | None = None,
+                           d_ef: DeviceBuffer | None = None, ef_max: int | None = None):
+        """Enqueue a fast-mode search (pk_search_fast_device); no wait."""
+        check(lib().pk_search_fast_device(self.h, d_queries.ptr, nq, d_k.ptr if d_k else None,
+                                          d_ef.ptr if d_ef else None, k, ef, ef_max if ef_max else ef, k_stride,
+                                          d_ids.ptr, d_dists.ptr, d_count.ptr if d_count else None,
+                                          d_ctr.ptr if d_ctr else None))
+
+    def set_fast_tuning(self, width: int = 0, hash_bits: int = 0, waves_per_group: int = 0,
+                        blocks_per_cu: int = 0) -> None:
+        check(lib().pk_set_fast_tuning(self.h, width, hash_bits, waves_per_group, blocks_per_cu))
+
+    def search_fast(self, queries: np.ndarray, k, ef, k_stride: int | None = None, counters: bool = False):
+        """Fast-mode search of host queries: (ids, dists, counts[, counters]).
+        k / ef may be ints or per-query arrays."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq = q.shape[0]
+        k_arr = None if np.isscalar(k) else np.ascontiguousarray(k, np.int32)
+        ef_arr = None if np.isscalar(ef) else np.ascontiguousarray(ef, np.int32)
+        if k_stride is None:
+            k_stride = int(k) if k_arr is None else int(k_arr.max())
+        ef_max = int(ef) if ef_arr is None else int(ef_arr.max())
+        bufs = [DeviceBuffer.from_array(q), DeviceBuffer(nq * k_stride * 4), DeviceBuffer(nq * k_stride * 4),
+                DeviceBuffer(nq * 4), DeviceBuffer(nq * COUNTER_DTYPE.itemsize) if counters else None,
+                DeviceBuffer.from_array(k_arr) if k_arr is not None else None,
+                DeviceBuffer.from_array(ef_arr) if ef_arr is not None else None]
+        d_q, d_i, d_d, d_c, d_t, d_k, d_e = bufs
+        try:
+            self.search_fast_device(d_q, nq, 0 if k_arr is not None else int(k), 0 if ef_arr is not None else int(ef),
+                                    k_stride, d_i, d_d, d_c, d_t, d_k, d_e, ef_max)
+            self.sync()
+            ids = d_i.to_array(np.int32, (nq, k_stride))
+            ds = d_d.to_array(np.float32, (nq, k_stride))
+            cnt = d_c.to_array(np.int32, (nq,))
+            ctr = d_t.to_array(COUNTER_DTYPE, (nq,)) if counters else None
+        finally:
+            for b in bufs:
+                if b is not None:
+                    b.free()
+        if counters:
+            return ids, ds, cnt, ctr
+        return ids, ds, cnt
 
     def init_build(self, levels: np.ndarray) -> None:
         """InitBuffer: levels for every external id of the capacity."""

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpickle_amd.so")
-SOURCES = ["search.hip", "index.hip", "graph_file.hip", "brute.hip", "brute_fp64.hip", "insert.hip", "precision.hip", "server.hip"]
+SOURCES = ["search.hip", "index.hip", "graph_file.hip", "brute.hip", "brute_fp64.hip", "insert.hip", "precision.hip", "server.hip", "fast.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -1,0 +1,616 @@
+// fast.hip -- batched HNSW search, fast mode: sorted beam in registers,
+// several expansions per round trip, visited filter in LDS.
+//
+// Reference: HNSWGraph::AnnSearch (include/hnsw.hpp:509-541): SlideLayer
+// (:446-494) above layer 0, SearchLayerFast (:374-444) on layer 0.  This
+// mode keeps the reference's accept rule (a neighbour enters the result set
+// W when |W| < ef or it is strictly closer than W's farthest entry) and its
+// stopping rule (the search ends when no entry of W is left to expand), and
+// changes how the work is organised:
+//
+//   * W is one sorted array of (distance, id, expanded) keys held in
+//     registers, 64 keys per register row, RR rows (ef <= 64 * RR).  The
+//     reference's separate candidate heap is not needed: a candidate that
+//     is not in W is never expanded by the reference either (its distance
+//     is at least W's farthest when it is popped, hnsw.hpp:403).
+//   * Each round expands the `width` closest unexpanded entries of W at
+//     once: their adjacency rows and the rows of every new neighbour are
+//     gathered in the same memory round trips.  Width 1 expands in the
+//     reference's order; a larger width trades a few extra expansions for
+//     fewer dependent round trips per query.
+//   * New neighbours that pass the accept rule are sorted with a wave
+//     bitonic network and merged into W with a bitonic merge (the 64
+//     smallest of W's last row and the sorted batch, then a merge of the
+//     whole array), keeping the ef smallest.
+//   * The visited filter is a hash table in LDS (2^hash_bits ids per wave)
+//     whose slots are overwritten on collision, so it may forget an id.  A
+//     forgotten id is only evaluated again: if it is still in W, the merge
+//     places both copies side by side and one is dropped (keeping the
+//     expanded mark); if it left W, its distance is at or above W's
+//     farthest and the accept rule rejects it.  Results therefore do not
+//     depend on the table size, only the work does.  No visited state goes
+//     to device memory.
+//
+// Ties between equal distances are ordered by id here, by heap layout in
+// the reference; without ties, width 1 returns the reference's results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pickle_gpu.h"
+#include "index_impl.hpp"
+#include "traverse.hpp"
+
+namespace pk {
+
+constexpr uint64_t kKeyInf = ~0ull;
+constexpr int kFastCap = 256;  // new neighbours gathered before a flush (distances + merge)
+
+struct FastArgs {
+    const LayerDev* layers;
+    const int32_t* ext0;
+    int32_t enter_level;
+    int32_t dim;
+    int32_t vstride;
+    int32_t qbytes;
+    const uint8_t* queries;
+    int64_t nq;
+    const int32_t* k;
+    const int32_t* ef;
+    int32_t k_all, ef_all, k_stride;
+    int32_t* out_ids;
+    float* out_dists;
+    int32_t* out_count;
+    pk_query_counters* ctr;
+    int32_t width;      // entries expanded per round (1..4)
+    int32_t hash_bits;  // log2 slots of the LDS visited filter per wave
+    int32_t lds_per_wave;
+    int32_t max_rounds;  // per query; past it the query fails (flag 1)
+    int32_t* qcounter;
+    int32_t* err;
+};
+
+// ---- keys: order-preserving distance bits, then id, then the pending bit
+
+__device__ __forceinline__ uint32_t dist_key(float d) {
+    const uint32_t u = __float_as_uint(d);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_dist(uint64_t k) {
+    const uint32_t ku = (uint32_t)(k >> 32);
+    return __uint_as_float((ku & 0x80000000u) ? (ku & 0x7fffffffu) : ~ku);
+}
+// bit 0 set: not expanded yet.  An expanded copy of an id therefore sorts
+// before a pending copy of it, and keeping the first of adjacent copies
+// keeps the expanded one.
+__device__ __forceinline__ uint64_t make_key(float d, int id) {
+    return ((uint64_t)dist_key(d) << 32) | ((uint64_t)(uint32_t)id << 1) | 1ull;
+}
+__device__ __forceinline__ bool pending(uint64_t k) { return (k & 1) && k != kKeyInf; }
+__device__ __forceinline__ int key_id(uint64_t k) { return (int)((uint32_t)k >> 1); }
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const int lo = __shfl((int)(uint32_t)v, src, kWave);
+    const int hi = __shfl((int)(uint32_t)(v >> 32), src, kWave);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m, kWave);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+// lane l's key, uniform (v_readlane reads any lane, whatever its exec bit)
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+__device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t max64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+// ascending sort of one key per lane
+__device__ __forceinline__ uint64_t bitonic_sort64(uint64_t x, int lane) {
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t y = shfl_xor64(x, j);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            x = (lower == up) ? min64(x, y) : max64(x, y);
+        }
+    }
+    return x;
+}
+
+// a bitonic sequence of one key per lane, sorted (ascending or descending)
+__device__ __forceinline__ uint64_t bitonic_merge64(uint64_t x, int lane, bool ascending) {
+#pragma unroll
+    for (int j = kWave >> 1; j > 0; j >>= 1) {
+        const uint64_t y = shfl_xor64(x, j);
+        const bool lower = (lane & j) == 0;
+        x = (lower == ascending) ? min64(x, y) : max64(x, y);
+    }
+    return x;
+}
+
+// The sorted beam: position p = 64 * r + lane lives in row r.
+template <int RR>
+struct Beam {
+    uint64_t key[RR];
+
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int r = 0; r < RR; r++) key[r] = kKeyInf;
+    }
+
+    // W <- the 64 * RR smallest keys of W and the sorted batch s (one key
+    // per lane): the smallest 64 of W's last row and s, sorted, then one
+    // bitonic merge of the whole array.  Returns the 64 keys that fall off
+    // the end, sorted: the merged sequence continues with them, so a
+    // duplicate dropped from W is replaced from there.
+    __device__ __forceinline__ uint64_t merge(uint64_t s, int lane) {
+        const uint64_t srev = shfl64(s, kWave - 1 - lane);
+        const uint64_t last = key[RR - 1];
+        const uint64_t t = min64(last, srev);
+        const uint64_t over = bitonic_merge64(max64(last, srev), lane, true);
+        if (RR == 1) {
+            key[0] = bitonic_merge64(t, lane, true);
+            return over;
+        }
+        key[RR - 1] = bitonic_merge64(t, lane, false);  // rows ascending, then this row descending
+#pragma unroll
+        for (int J = RR / 2; J >= 1; J >>= 1)
+#pragma unroll
+            for (int r = 0; r < RR; r++)
+                if ((r & J) == 0) {
+                    const uint64_t a = key[r], b = key[r + J];
+                    key[r] = min64(a, b);
+                    key[r + J] = max64(a, b);
+                }
+#pragma unroll
+        for (int r = 0; r < RR; r++) key[r] = bitonic_merge64(key[r], lane, true);
+        return over;
+    }
+
+    // Over W followed by the overflow row (the sorted continuation from
+    // merge): keeps the first of adjacent copies of one (distance, id) --
+    // the expanded one if any, since it sorts first -- and refills W from
+    // the overflow; returns the copies dropped.  Survivors move left through
+    // LDS (64 * (RR + 1) keys of scratch) -- only when an id the visited
+    // filter forgot came back while still in W.
+    __device__ int dedup(uint64_t over, uint64_t* scratch, int lane) {
+        uint64_t v[RR + 1];
+#pragma unroll
+        for (int r = 0; r < RR; r++) v[r] = key[r];
+        v[RR] = over;
+        uint64_t dupm[RR + 1];
+        int total = 0;
+#pragma unroll
+        for (int r = 0; r <= RR; r++) {
+            const uint64_t row_before = r > 0 ? readlane64(v[r - 1], kWave - 1) : kKeyInf;
+            uint64_t prev = shfl64(v[r], lane > 0 ? lane - 1 : 0);
+            if (lane == 0) prev = row_before;
+            const bool dup = v[r] != kKeyInf && prev != kKeyInf && (v[r] >> 1) == (prev >> 1);
+            dupm[r] = ballot(dup);
+            if (r < RR) total += __builtin_popcountll(dupm[r]);
+        }
+        // a duplicate inside the overflow alone changes nothing in W
+        if (total == 0 && !(dupm[RR] & 1)) return 0;
+        int before = 0, all = 0;
+#pragma unroll
+        for (int r = 0; r <= RR; r++) {
+            const bool dup = (dupm[r] >> lane) & 1;
+            const int np = 64 * r + lane - before - lanes_below(dupm[r]);
+            if (!dup) scratch[np] = v[r];
+            before += __builtin_popcountll(dupm[r]);
+        }
+        all = before;
+        for (int p = 64 * (RR + 1) - all + lane; p < 64 * (RR + 1); p += kWave) scratch[p] = kKeyInf;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < RR; r++) key[r] = scratch[64 * r + lane];
+        __builtin_amdgcn_wave_barrier();
+        return total;
+    }
+
+    // positions >= ef leave the beam
+    __device__ __forceinline__ void cap(int ef, int lane) {
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+            if (64 * r + lane >= ef) key[r] = kKeyInf;
+    }
+
+    __device__ __forceinline__ int count() const {
+        int n = 0;
+#pragma unroll
+        for (int r = 0; r < RR; r++) n += __builtin_popcountll(ballot(key[r] != kKeyInf));
+        return n;
+    }
+
+    __device__ __forceinline__ uint64_t at(int p) const {
+        uint64_t v = kKeyInf;
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+            if (r == (p >> 6)) v = readlane64(key[r], p & 63);
+        return v;
+    }
+};
+
+// LDS visited filter: test-and-set by exchange; a collision overwrites
+__device__ __forceinline__ bool filter_new(uint32_t* tab, int bits, int id) {
+    const uint32_t slot = ((uint32_t)id * 0x9E3779B1u) >> (32 - bits);
+    const uint32_t old = atomicExch(&tab[slot], (uint32_t)id + 1u);
+    return old != (uint32_t)id + 1u;
+}
+
+// SlideLayer without a visited set (a node seen before cannot be strictly
+// closer than the current one): move to the strictly closest neighbour,
+// first in row order, until none is closer.
+template <class DF>
+__device__ int greedy_fast(const DF& df, const LayerDev& L, int vstride, int cur, int* cl, float* cd, int lane,
+                           WorkCount& wc) {
+    float cdist = one_distance(df, L, vstride, cur, cl, cd, lane);
+    wc.dists++;
+    for (int guard = 0; guard < (1 << 20); guard++) {
+        const RowHead rh = load_row_head(L, cur, lane);
+        wc.hops++;
+        wc.neighbors += rh.deg;
+        int best = cur;
+        float bestd = cdist;
+        for (int b0 = 0; b0 < rh.deg; b0 += kWave) {
+            const int j = b0 + lane;
+            const bool inrow = j < rh.deg;
+            const int id = b0 == 0 ? rh.first : (inrow ? gbl(L.adj)[rh.base + j] : -1);
+            const bool valid = inrow && id >= 0 && (int64_t)id < L.n;
+            const uint64_t vm = ballot(valid);
+            const int pos = lanes_below(vm);
+            if (valid) cl[pos] = id;
+            __builtin_amdgcn_wave_barrier();
+            const int nc = __builtin_popcountll(vm);
+            batch_distances(df, L.vec, vstride, cl, cd, nc, lane);
+            __builtin_amdgcn_wave_barrier();
+            wc.dists += nc;
+            float m;
+            const float d = lane < nc ? cd[lane] : __builtin_inff();
+            const int at = wave_min_lane(d, lane < nc, &m);
+            if (at >= 0 && m < bestd) {
+                bestd = m;
+                best = readlane(lane < nc ? cl[lane] : 0, at);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (!(bestd < cdist)) break;
+        cur = best;
+        cdist = bestd;
+    }
+    return cur;
+}
+
+// Distances of the gathered new neighbours cl[0, n), then the accepted
+// ones merged into the beam, 64 at a time.
+template <class DF, int RR>
+__device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, float* cd, int n, Beam<RR>& W, int ef,
+                      int& len, uint64_t* scratch, int lane) {
+    batch_distances(df, L.vec, vstride, cl, cd, n, lane);
+    __builtin_amdgcn_wave_barrier();
+    for (int c0 = 0; c0 < n; c0 += kWave) {
+        const int c = c0 + lane;
+        uint64_t key = c < n ? make_key(cd[c], cl[c]) : kKeyInf;
+        if (len >= ef) {
+            const uint32_t worst = (uint32_t)(W.at(ef - 1) >> 32);  // hnsw.hpp:425: top > dist
+            if (!((uint32_t)(key >> 32) < worst)) key = kKeyInf;
+        }
+        if (ballot(key != kKeyInf) == 0) continue;
+        const uint64_t over = W.merge(bitonic_sort64(key, lane), lane);
+        W.dedup(over, scratch, lane);
+        W.cap(ef, lane);
+        len = W.count();
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <class DF, bool IS_INT, int RR>
+__global__ __launch_bounds__(256) void k_search_fast(FastArgs A) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = lane_id();
+    const int w = threadIdx.x >> 6;
+    uint8_t* my = smem + (size_t)w * A.lds_per_wave;
+    uint32_t* vtab = reinterpret_cast<uint32_t*>(my);
+    const int nslot = 1 << A.hash_bits;
+    int* cl = reinterpret_cast<int*>(vtab + nslot);
+    float* cd = reinterpret_cast<float*>(cl + kFastCap);
+    uint64_t* scratch = reinterpret_cast<uint64_t*>(cd + kFastCap);  // 64 * (RR + 1) keys
+    uint8_t* qst = reinterpret_cast<uint8_t*>(scratch + 64 * (RR + 1));
+    const bool leader = lane == 0;
+    const int g = lane % DF::LPC;
+
+    for (;;) {
+        int qi = 0;
+        if (leader) qi = atomicAdd(A.qcounter, 1);
+        const int64_t q = readlane(qi, 0);
+        if (q >= A.nq) break;
+        const uint64_t t_begin = wall_clock64();
+        stage_row(qst, A.queries + (size_t)q * (size_t)A.qbytes, A.qbytes, A.vstride, lane);
+        DF df;
+        df.load_query(qst, g, IS_INT ? (A.vstride >> 4) : A.dim);
+        const int k = uni(A.k ? A.k[q] : A.k_all);
+        const int ef = uni(A.ef ? A.ef[q] : A.ef_all);
+        int fail = 0;
+        if (k < 1 || k > A.k_stride || ef < 1 || ef > 64 * RR) fail = kFailArgs;
+
+        WorkCount up, l0;
+        int cur = 0;  // m_ent_id of the top layer (include/hnsw_layer.hpp:70, 186-188)
+        for (int level = A.enter_level; level >= 1 && !fail; level--) {
+            const LayerDev L = A.layers[level];
+            cur = greedy_fast(df, L, A.vstride, cur, cl, cd, lane, up);
+            cur = gbl(L.down)[cur];
+        }
+
+        Beam<RR> W;
+        W.clear();
+        int len = 0;
+        if (!fail) {
+            // an empty filter for this query
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            for (int i = lane; i < nslot / 4; i += kWave)
+                reinterpret_cast<u32x4*>(vtab)[i] = u32x4{0u, 0u, 0u, 0u};
+            __builtin_amdgcn_wave_barrier();
+            const LayerDev L = A.layers[0];
+            const float d0 = one_distance(df, L, A.vstride, cur, cl, cd, lane);
+            l0.dists++;
+            if (leader) {
+                W.key[0] = make_key(d0, cur);
+                filter_new(vtab, A.hash_bits, cur);
+            }
+            len = 1;
+            __builtin_amdgcn_wave_barrier();
+            int rounds = 0;
+            for (;;) {
+                // the `width` closest unexpanded entries, marked expanded
+                int sel[4];
+                int nsel = 0;
+#pragma unroll
+                for (int r = 0; r < RR; r++) {
+                    uint64_t m = ballot(pending(W.key[r]));
+                    while (m && nsel < A.width) {
+                        const int ln = __builtin_ctzll(m);
+                        m &= m - 1;
+                        const uint64_t kk = readlane64(W.key[r], ln);
+                        if (lane == ln) W.key[r] &= ~1ull;
+                        sel[nsel == 0 ? 0 : nsel == 1 ? 1 : nsel == 2 ? 2 : 3] = key_id(kk);
+                        nsel++;
+                    }
+                }
+                if (nsel == 0) break;
+                if (++rounds > A.max_rounds) {
+                    fail = kFailVisitedFull;
+                    break;
+                }
+                l0.hops += nsel;
+                // adjacency rows of the selected entries, new ids gathered
+                int n = 0;
+                for (int s = 0; s < nsel; s++) {
+                    const int v = s == 0 ? sel[0] : s == 1 ? sel[1] : s == 2 ? sel[2] : sel[3];
+                    const RowHead rh = load_row_head(L, v, lane);
+                    l0.neighbors += rh.deg;
+                    for (int b0 = 0; b0 < rh.deg; b0 += kWave) {
+                        if (n + kWave > kFastCap) {
+                            flush(df, L, A.vstride, cl, cd, n, W, ef, len, scratch, lane);
+                            l0.dists += n;
+                            n = 0;
+                        }
+                        const int j = b0 + lane;
+                        const bool inrow = j < rh.deg;
+                        const int id = b0 == 0 ? rh.first : (inrow ? gbl(L.adj)[rh.base + j] : -1);
+                        const bool valid = inrow && id >= 0 && (int64_t)id < L.n;
+                        const bool isnew = valid && filter_new(vtab, A.hash_bits, id);
+                        const uint64_t nm = ballot(isnew);
+                        if (isnew) cl[n + lanes_below(nm)] = id;
+                        n += __builtin_popcountll(nm);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (n > 0) {
+                    flush(df, L, A.vstride, cl, cd, n, W, ef, len, scratch, lane);
+                    l0.dists += n;
+                }
+            }
+        }
+
+        // results: the k closest, ascending, external ids
+        int32_t* oid = A.out_ids + (size_t)q * A.k_stride;
+        float* odist = A.out_dists + (size_t)q * A.k_stride;
+        const int kk = fail ? 0 : (k < len ? k : len);
+#pragma unroll
+        for (int r = 0; r < RR; r++) {
+            const int p = 64 * r + lane;
+            if (p < kk) {
+                oid[p] = A.ext0[key_id(W.key[r])];
+                odist[p] = key_dist(W.key[r]);
+            }
+        }
+        for (int i = kk + lane; i < A.k_stride; i += kWave) {
+            oid[i] = -1;
+            odist[i] = __builtin_inff();
+        }
+        if (leader) {
+            if (A.out_count) A.out_count[q] = fail ? -1 : kk;
+            if (fail) atomicOr(A.err, fail);
+            if (A.ctr) {
+                pk_query_counters c;
+                c.hops[0] = l0.hops;
+                c.hops[1] = up.hops;
+                c.neighbors[0] = l0.neighbors;
+                c.neighbors[1] = up.neighbors;
+                c.dists[0] = l0.dists;
+                c.dists[1] = up.dists;
+                c.latency_ticks = (int32_t)(wall_clock64() - t_begin);
+                A.ctr[q] = c;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void k_fill_i32_fast(int* p, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <class DF, bool INT>
+static hipError_t launch_rr(const FastArgs& a, int rr, int grid, int block, size_t lds, hipStream_t s) {
+    switch (rr) {
+        case 1: hipLaunchKernelGGL((k_search_fast<DF, INT, 1>), dim3(grid), dim3(block), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((k_search_fast<DF, INT, 2>), dim3(grid), dim3(block), lds, s, a); break;
+        case 4: hipLaunchKernelGGL((k_search_fast<DF, INT, 4>), dim3(grid), dim3(block), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_search_fast<DF, INT, 8>), dim3(grid), dim3(block), lds, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <int DT, int MET>
+static hipError_t launch_fast_dt(const FastArgs& a, int size_class, int rr, int grid, int block, size_t lds,
+                                 hipStream_t s) {
+    constexpr bool INT = DT == DT_U8 || DT == DT_I8;
+    if constexpr (INT) {
+        if (size_class == 0) return launch_rr<IntDist<DT, MET, 1>, true>(a, rr, grid, block, lds, s);
+        if (size_class == 1) return launch_rr<IntDist<DT, MET, 2>, true>(a, rr, grid, block, lds, s);
+        return launch_rr<IntDist<DT, MET, 4>, true>(a, rr, grid, block, lds, s);
+    } else {
+        if constexpr (DT == DT_F16) {
+            if (size_class == kHalfFastClass) return launch_rr<HalfDist<MET, 1>, true>(a, rr, grid, block, lds, s);
+            if (size_class == kHalfFastClass + 1) return launch_rr<HalfDist<MET, 2>, true>(a, rr, grid, block, lds, s);
+        }
+        if (size_class == 0) return launch_rr<FloatDist<DT, MET, 8>, false>(a, rr, grid, block, lds, s);
+        return launch_rr<FloatDist<DT, MET, 16>, false>(a, rr, grid, block, lds, s);
+    }
+}
+
+static hipError_t launch_fast(const FastArgs& a, int dtype, int metric, int size_class, int rr, int grid, int block,
+                              size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32_fast, dim3(1), dim3(64), 0, s, a.qcounter, 0, 1);
+    hipLaunchKernelGGL(k_fill_i32_fast, dim3(1), dim3(64), 0, s, a.err, 0, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (dtype * 4 + metric) {
+        case DT_U8 * 4 + MET_L2: return launch_fast_dt<DT_U8, MET_L2>(a, size_class, rr, grid, block, lds, s);
+        case DT_U8 * 4 + MET_IP: return launch_fast_dt<DT_U8, MET_IP>(a, size_class, rr, grid, block, lds, s);
+        case DT_I8 * 4 + MET_L2: return launch_fast_dt<DT_I8, MET_L2>(a, size_class, rr, grid, block, lds, s);
+        case DT_I8 * 4 + MET_IP: return launch_fast_dt<DT_I8, MET_IP>(a, size_class, rr, grid, block, lds, s);
+        case DT_F32 * 4 + MET_L2: return launch_fast_dt<DT_F32, MET_L2>(a, size_class, rr, grid, block, lds, s);
+        case DT_F32 * 4 + MET_IP: return launch_fast_dt<DT_F32, MET_IP>(a, size_class, rr, grid, block, lds, s);
+        case DT_F16 * 4 + MET_L2: return launch_fast_dt<DT_F16, MET_L2>(a, size_class, rr, grid, block, lds, s);
+        case DT_F16 * 4 + MET_IP: return launch_fast_dt<DT_F16, MET_IP>(a, size_class, rr, grid, block, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// rows of the beam for an ef: 64, 128, 256 or 512 keys
+static int beam_rows(int ef_max) { return ef_max <= 64 ? 1 : ef_max <= 128 ? 2 : ef_max <= 256 ? 4 : 8; }
+
+static hipError_t fast_occupancy(int dtype, int rr, int block, size_t lds, int* blocks_per_cu) {
+    const void* fn = nullptr;
+    const bool in = dtype == DT_U8 || dtype == DT_I8;
+#define PK_FN(R) \
+    (in ? reinterpret_cast<const void*>(&k_search_fast<IntDist<DT_U8, MET_L2, 1>, true, R>) \
+        : reinterpret_cast<const void*>(&k_search_fast<FloatDist<DT_F32, MET_L2, 8>, false, R>))
+    switch (rr) {
+        case 1: fn = PK_FN(1); break;
+        case 2: fn = PK_FN(2); break;
+        case 4: fn = PK_FN(4); break;
+        default: fn = PK_FN(8); break;
+    }
+#undef PK_FN
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block, lds);
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+namespace pk {
+int search_class_of(const pk_index* h);
+int ensure_search_counters(pk_index* h);
+}
+
+extern "C" {
+
+int pk_set_fast_tuning(pk_index* h, int width, int hash_bits, int waves_per_group, int blocks_per_cu) {
+    if (!h) return set_error(PK_ERR_ARG, "null handle");
+    if (h->servers_alive) return set_error(PK_ERR_STATE, "destroy the serving loops before changing the tuning");
+    if (width < 0 || width > 4 || (hash_bits != 0 && (hash_bits < 8 || hash_bits > 14)) ||
+        (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4) ||
+        blocks_per_cu < 0)
+        return set_error(PK_ERR_ARG, "bad fast tuning (width 0..4, hash_bits 0 or 8..14, waves_per_group 0/1/2/4)");
+    h->fast.width = width;
+    h->fast.hash_bits = hash_bits;
+    h->fast.waves_per_group = waves_per_group;
+    h->fast.blocks_per_cu = blocks_per_cu;
+    return PK_OK;
+}
+
+int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                          int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                          float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (!h) return set_error(PK_ERR_ARG, "null handle");
+    if (!h->ready) return set_error(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
+    if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return set_error(PK_ERR_ARG, "bad buffers");
+    if (k_stride < 1) return set_error(PK_ERR_ARG, "k_stride must be positive");
+    if (!d_k && (k_all < 1 || k_all > k_stride)) return set_error(PK_ERR_ARG, "k must be in [1, k_stride]");
+    if (!d_ef && (ef_all < 1 || ef_all > ef_max)) return set_error(PK_ERR_ARG, "ef must be in [1, ef_max]");
+    if (ef_max < 1 || ef_max > 512) return set_error(PK_ERR_ARG, "fast mode takes ef_max in [1, 512]");
+    if (nq > (int64_t)1 << 27) return set_error(PK_ERR_CAPACITY, "at most 2^27 queries per call");
+    if (nq == 0) return PK_OK;
+    int rc = ensure_search_counters(h);
+    if (rc) return rc;
+    const FastTuning& t = h->fast;
+    const int rr = beam_rows(ef_max);
+    const int width = t.width > 0 ? t.width : 2;
+    const int hb = t.hash_bits > 0 ? t.hash_bits : 12;
+    const int wpg = t.waves_per_group > 0 ? t.waves_per_group : 2;
+    size_t per_wave = ((size_t)4 << hb) + (size_t)kFastCap * 8 + (size_t)64 * (rr + 1) * 8 + h->vstride;
+    per_wave = (per_wave + 15) / 16 * 16;
+    const size_t lds = per_wave * wpg;
+    if (lds > 160 * 1024) return set_error(PK_ERR_CAPACITY, "fast search needs too much LDS per workgroup");
+    int bpc = 0;
+    if (fast_occupancy(h->dtype, rr, 64 * wpg, lds, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    if (t.blocks_per_cu > 0) bpc = std::min(bpc, t.blocks_per_cu);
+    const int64_t want = (nq + wpg - 1) / wpg;
+    const int grid = (int)std::min<int64_t>(want, (int64_t)h->num_cus * bpc);
+    FastArgs a{};
+    a.layers = static_cast<const LayerDev*>(h->d_layers);
+    a.ext0 = h->layers[0].ext;
+    a.enter_level = h->enter_level;
+    a.dim = h->dim;
+    a.vstride = h->vstride;
+    a.qbytes = h->qbytes;
+    a.queries = static_cast<const uint8_t*>(d_queries);
+    a.nq = nq;
+    a.k = d_k;
+    a.ef = d_ef;
+    a.k_all = k_all;
+    a.ef_all = ef_all;
+    a.k_stride = k_stride;
+    a.out_ids = d_out_ids;
+    a.out_dists = d_out_dists;
+    a.out_count = d_out_count;
+    a.ctr = d_counters;
+    a.width = width;
+    a.hash_bits = hb;
+    a.lds_per_wave = (int)per_wave;
+    a.max_rounds = (int)std::min<int64_t>(1 << 30, 4 * h->capacity + 64);
+    a.qcounter = h->sc.qcounter;
+    a.err = h->sc.err;
+    const hipError_t e =
+        launch_fast(a, h->dtype, h->metric, search_class_of(h), rr, grid, 64 * wpg, lds, h->stream);
+    if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("fast search launch: ") + hipGetErrorString(e));
+    h->last_grid = grid;
+    h->last_lds = lds;
+    return PK_OK;
+}
+
+}  // extern "C"

@@ -563,6 +563,17 @@ static int search_class(const pk_index* h) {
 // rerun of queries that filled their tables); such a launch takes no more
 // waves than the scratch already allocated holds at that size, since the
 // persistent queue serves any number of queries with fewer waves.
+// the size class a search on h runs with (fast.hip)
+int search_class_of(const pk_index* h) { return search_class(h); }
+
+// the work counter and failure flags of the handle's own search context
+int ensure_search_counters(pk_index* h) {
+    int rc;
+    if (!h->sc.qcounter && (rc = dalloc(&h->sc.qcounter, 1))) return rc;
+    if (!h->sc.err && (rc = dalloc(&h->sc.err, 1))) return rc;
+    return PK_OK;
+}
+
 static int plan_search(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                        const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
                        int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,

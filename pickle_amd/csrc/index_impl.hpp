@@ -56,6 +56,14 @@ struct SearchTuning {
     int blocks_per_cu = 0;  // cap on resident workgroups per CU (0 = occupancy)
 };
 
+// fast-mode search (fast.hip): 0 = automatic for every field
+struct FastTuning {
+    int width = 0;            // entries expanded per round (1..4)
+    int hash_bits = 0;        // log2 slots of each wave's LDS visited filter (8..14)
+    int waves_per_group = 0;  // queries per workgroup (1, 2, 4)
+    int blocks_per_cu = 0;    // cap on resident workgroups per CU
+};
+
 // search scratch bound to one stream: per resident wave a candidate-heap
 // spill area, a visited table and its log; the work counter and failure flags
 struct SearchCtx {
@@ -130,6 +138,7 @@ struct pk_index {
     pk::SearchCtx sc;
     int cg_cap = 16384;
     pk::SearchTuning tuning;
+    pk::FastTuning fast;
     pk::IoBuf io[8];
     int last_grid = 0;
     size_t last_lds = 0;

@@ -51,8 +51,11 @@ WORKLOADS = {
     "gm100m_u8": dict(desc="100M x 128 uint8, 1024-cluster gaussian mixture quantised to [0,255] (seed 4), L2, "
                            "M=32, efC=256", n=100_000_000, nq=100_000, build="gpu"),
     "gm10m_f32": dict(desc="10M x 128 float32, 64-cluster gaussian mixture (seed 3), L2, M=32, efC=200, 100k "
-                           "queries, k=10, GPU-built graph, fp16 traversal copy + FP32 re-rank",
-                      n=10_000_000, nq=100_000, build="gpu", precision="f16"),
+                           "queries, k=10, GPU-built graph (reference rows up to 128 ids); FP32 exact and fast legs, "
+                           "FP16 and per-dimension int8 traversal copies + FP32 re-rank",
+                      n=10_000_000, nq=100_000, build="gpu", precision="f16,i8",
+                      efs="32,64,96,128,160,192,224,256,288,320,352,384,448,512", gt_queries=10_000,
+                      cpu_queries=10_000, row_limit=128),
 }
 
 
@@ -60,11 +63,47 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cgroup_cpus():
+    """CPUs the cgroup's quota allows (cpu.max), or None without a quota."""
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            return max(1, int(int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def host_threads() -> int:
+    """Every core this process may use: its CPU affinity, bounded by the
+    cgroup quota (OMP_NUM_THREADS overrides)."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
         return int(env)
-    return min(os.cpu_count() or 1, 16)
+    n = len(os.sched_getaffinity(0))
+    q = cgroup_cpus()
+    return min(n, q) if q else n
+
+
+def cpu_info() -> dict:
+    """lscpu's model, sockets, cores and threads, plus what this process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    sockets = int(info.get("Socket(s)", "1") or 1)
+    cps = int(info.get("Core(s) per socket", "0") or 0)
+    tpc = int(info.get("Thread(s) per core", "1") or 1)
+    return {"model": info.get("Model name", cpu_model()), "sockets": sockets, "cores": sockets * cps,
+            "threads": sockets * cps * tpc, "online_cpus": info.get("CPU(s)"),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "cgroup_cpu_quota": cgroup_cpus(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "avx512": has_avx512()}
 
 
 def cpu_model() -> str:
@@ -130,14 +169,17 @@ def read_ref_results(path, nq, k):
 # rocprofv3 summaries (profiles/) and the traffic they record
 # --------------------------------------------------------------------------
 
-def pmc_profile_path(workload, ef):
-    return os.path.join(PROFILES, f"k_search_{workload}_ef{ef}_pmc.md")
+KERNEL_OF_LEG = {"exact": "k_search", "fast": "k_search_fast"}
 
 
-def profiled_traffic(workload, n, nq, k, ef):
-    """HBM bytes per k_search launch and its L2 hit rate, recorded by a
-    --summarize-pmc run of the same workload, size and ef (None if none)."""
-    path = pmc_profile_path(workload, ef)
+def pmc_profile_path(workload, ef, leg="exact"):
+    return os.path.join(PROFILES, f"{KERNEL_OF_LEG[leg]}_{workload}_ef{ef}_pmc.md")
+
+
+def profiled_traffic(workload, n, nq, k, ef, leg="exact"):
+    """HBM bytes per launch of the leg's kernel and its L2 hit rate, recorded
+    by a --summarize-pmc run of the same workload, size and ef (None if none)."""
+    path = pmc_profile_path(workload, ef, leg)
     if not os.path.exists(path):
         return None, None, None
     for line in open(path):
@@ -162,18 +204,23 @@ def _csv_rows(directory, suffix):
 
 
 def _short_kernel(name):
-    for key in ("k_search", "k_brute_imma", "k_brute_dmma", "k_brute_rowconst", "k_brute_rownorm", "k_brute_merge",
+    for key in ("k_search_fast", "k_search", "k_brute_imma", "k_brute_dmma", "k_brute_rowconst", "k_brute_rownorm", "k_brute_merge",
                 "k_brute_fill",
                 "k_copy16", "k_gather_rows", "k_fill_i32", "k_insert_search", "k_insert_rows", "k_insert_edges",
-                "k_rerank"):
+                "k_rerank", "k_quantize"):
         if key in name:
             return key + (name[name.index("<"):name.index(">") + 1] if "<" in name and key != "k_copy16" else "")
     return name[:60]
 
 
-def _counter_per_dispatch(directory, counter, last=None):
+def _is_kernel(short, kernel):
+    """short names carry template arguments: k_search<...> is not k_search_fast<...>."""
+    return short == kernel or short.startswith(kernel + "<")
+
+
+def _counter_per_dispatch(directory, counter, last=None, kernel="k_search"):
     """Counter values per kernel, in dispatch order; with `last`, only the
-    last `last` dispatches of k_search (the timed region of a bench run)."""
+    last `last` dispatches of `kernel` (the timed region of a bench run)."""
     rows = []
     for r in _csv_rows(directory, "counter_collection.csv"):
         if r["Counter_Name"] == counter:
@@ -184,14 +231,14 @@ def _counter_per_dispatch(directory, counter, last=None):
         out.setdefault(key, []).append(v)
     if last:
         for key in out:
-            if key.startswith("k_search"):
+            if _is_kernel(key, kernel):
                 out[key] = out[key][-last:]
     return out
 
 
-def _timed_search_ms(directory, last):
-    """Average duration of the last `last` k_search dispatches in a kernel trace."""
-    rows = [r for r in _csv_rows(directory, "kernel_trace.csv") if "k_search" in r["Kernel_Name"]]
+def _timed_search_ms(directory, last, kernel="k_search"):
+    """Average duration of the last `last` dispatches of `kernel` in a kernel trace."""
+    rows = [r for r in _csv_rows(directory, "kernel_trace.csv") if _is_kernel(_short_kernel(r["Kernel_Name"]), kernel)]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     sel = rows[-last:]
     return sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sel) / len(sel) / 1e6
@@ -202,7 +249,8 @@ def summarize(args):
     the same command."""
     bl = json.loads(open(args.bench_json).read().strip().splitlines()[-1])
     cfg, roof = bl["config"], bl["roofline"]
-    head = [f"# k_search — {cfg['workload']}", "",
+    kern = roof.get("kernel", "k_search")
+    head = [f"# {kern} — {cfg['workload']}", "",
             f"ef={cfg['ef']}, k={cfg['k']}, {cfg['nq']} queries per launch, recall@10 = {cfg['recall@10']}", "",
             "Commands (committed files only):", ""]
     head += ["    " + c for c in args.cmd]
@@ -215,58 +263,64 @@ def summarize(args):
         for r in rows:
             avg = float(r["AverageNs"]) / 1e6
             name = _short_kernel(r["Name"])
-            if name.startswith("k_search"):
+            if _is_kernel(name, kern):
                 search_avg = avg
             body.append(f"| `{name}` | {r['Calls']} | {avg:.4f} ms | {float(r['MinNs']) / 1e6:.4f} ms | "
                         f"{float(r['MaxNs']) / 1e6:.4f} ms |")
-        timed_ms = _timed_search_ms(args.summarize_trace, bl["steps"])
-        body += ["", f"The last {bl['steps']} k_search dispatches are the timed region: {timed_ms:.4f} ms each on "
-                 f"average in the trace (the others are the ef sweep, the counter pass and the autotuner's trials).",
-                 f"bench.py in the same run: {roof['kernel_ms']} ms per k_search launch (HIP events on the "
+        timed_ms = _timed_search_ms(args.summarize_trace, bl["steps"], kern)
+        apq = roof.get("algorithmic_per_query", roof.get("per_query"))
+        body += ["", f"The last {bl['steps']} {kern} dispatches are the timed region: {timed_ms:.4f} ms each on "
+                 f"average in the trace (the others are the ef sweeps, the counter pass and the autotuner's trials).",
+                 f"bench.py in the same run: {roof['kernel_ms']} ms per {kern} launch (HIP events on the "
                  f"index stream), {bl['value']:,.0f} QPS, algorithmic "
                  f"{roof['algorithmic_bytes_per_launch'] / 1e9:.3f} GB per launch -> {roof['achieved']} GB/s "
                  f"= {roof['frac'] * 100:.2f} % of 8 TB/s.",
-                 f"Per query: {roof['per_query']['hops']} hops, {roof['per_query']['neighbors']} adjacency "
-                 f"entries scanned, {roof['per_query']['dists']} distances."]
+                 f"Algorithmic work per query (reference Profiler units): {apq['hops']} hops, {apq['neighbors']} "
+                 f"adjacency entries scanned, {apq['dists']} distances."]
+        if "kernel_per_query" in roof:
+            kq = roof["kernel_per_query"]
+            body += [f"The kernel's own counters per query: {kq['hops']} hops, {kq['dists']} distances "
+                     f"(ids its LDS visited filter forgot are read again): "
+                     f"{roof['kernel_counted_bytes_per_launch'] / 1e9:.3f} GB per launch."]
     if args.summarize_pmc:
         fdir, wdir = args.summarize_pmc
-        fetch = _counter_per_dispatch(fdir, "FETCH_SIZE", last=bl["steps"])
-        write = _counter_per_dispatch(wdir, "WRITE_SIZE", last=bl["steps"])
+        fetch = _counter_per_dispatch(fdir, "FETCH_SIZE", last=bl["steps"], kernel=kern)
+        write = _counter_per_dispatch(wdir, "WRITE_SIZE", last=bl["steps"], kernel=kern)
         kib = CALIB_BYTES / 1024.0
         ff = (sum(fetch["k_copy16"]) / len(fetch["k_copy16"])) / kib
         fw = (sum(write["k_copy16"]) / len(write["k_copy16"])) / kib
-        skey = [key for key in fetch if key.startswith("k_search")][0]
+        skey = [key for key in fetch if _is_kernel(key, kern)][0]
         f_kib = sum(fetch[skey]) / len(fetch[skey])
         w_kib = sum(write[skey]) / len(write[skey])
         fb, wb = f_kib * 1024 / ff, w_kib * 1024 / fw
         alg = roof["algorithmic_bytes_per_launch"]
         body += ["", "Hardware byte counters (two `--pmc` passes, KiB per dispatch, averaged over the "
-                 "dispatches of each kernel; for k_search the timed region's last dispatches):", "",
+                 f"dispatches of each kernel; for {kern} the timed region's last dispatches):", "",
                  "| Kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |", "|---|---|---|---|"]
         for key in sorted(set(fetch) | set(write)):
             fv, wv = fetch.get(key, [0]), write.get(key, [0])
             body.append(f"| `{key}` | {len(fv)} | {sum(fv) / len(fv):,.1f} | {sum(wv) / len(wv):,.1f} |")
         body += ["", f"Calibration: `k_copy16` reads and writes {CALIB_BYTES >> 20} MiB "
                  f"({kib:,.0f} KiB); FETCH_SIZE reads {ff:.4f} x and WRITE_SIZE {fw:.4f} x that.",
-                 f"k_search per launch: {fb / 1e9:.3f} GB fetched + {wb / 1e9:.3f} GB written = "
+                 f"{kern} per launch: {fb / 1e9:.3f} GB fetched + {wb / 1e9:.3f} GB written = "
                  f"{(fb + wb) / 1e9:.3f} GB of HBM traffic, against {alg / 1e9:.3f} GB algorithmic "
                  f"({(fb + wb) / alg:.2f} x).", ""]
-        rec = {"workload": args.workload, "n": cfg["n"], "nq": cfg["nq"], "k": cfg["k"], "ef": cfg["ef"],
+        rec = {"kernel": kern, "workload": args.workload, "n": cfg["n"], "nq": cfg["nq"], "k": cfg["k"], "ef": cfg["ef"],
                "fetch_bytes": int(fb), "write_bytes": int(wb), "traffic_bytes": int(fb + wb),
                "algorithmic_bytes": alg, "factor_fetch": round(ff, 5), "factor_write": round(fw, 5)}
         if args.summarize_l2:
-            hits = _counter_per_dispatch(args.summarize_l2, "TCC_HIT_sum", last=bl["steps"])
-            miss = _counter_per_dispatch(args.summarize_l2, "TCC_MISS_sum", last=bl["steps"])
+            hits = _counter_per_dispatch(args.summarize_l2, "TCC_HIT_sum", last=bl["steps"], kernel=kern)
+            miss = _counter_per_dispatch(args.summarize_l2, "TCC_MISS_sum", last=bl["steps"], kernel=kern)
             body += ["L2 (TCC) requests, a third `--pmc` pass (TCC_HIT_sum, TCC_MISS_sum per dispatch):", "",
                      "| Kernel | hits | misses | hit rate |", "|---|---|---|---|"]
             for key in sorted(set(hits) & set(miss)):
                 hv, mv = sum(hits[key]) / len(hits[key]), sum(miss[key]) / len(miss[key])
                 rate = hv / (hv + mv) if hv + mv else 0.0
                 body.append(f"| `{key}` | {hv:,.0f} | {mv:,.0f} | {rate * 100:.1f} % |")
-            hk = [key for key in hits if key.startswith("k_search")][0]
+            hk = [key for key in hits if _is_kernel(key, kern)][0]
             hv, mv = sum(hits[hk]) / len(hits[hk]), sum(miss[hk]) / len(miss[hk])
             rec["l2_hit_rate"] = round(hv / (hv + mv), 4)
-            body += ["", f"k_search L2 hit rate: {rec['l2_hit_rate'] * 100:.1f} % of its L2 requests.", ""]
+            body += ["", f"{kern} L2 hit rate: {rec['l2_hit_rate'] * 100:.1f} % of its L2 requests.", ""]
         body.append("traffic-json: " + json.dumps(rec))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
@@ -597,37 +651,44 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="gm1m_u8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="gm10m_f32", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="override the base size (smaller runs)")
     ap.add_argument("--nq", type=int, default=None)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--recall-target", type=float, default=0.95)
-    ap.add_argument("--efs", default="10,12,14,16,20,24,28,32,40,48,64,96,128,192,256,384,512")
+    ap.add_argument("--efs", default=None, help="ef sweep (default per workload)")
+    ap.add_argument("--mode", choices=["fast", "exact"], default="fast",
+                    help="the headline leg: fast mode or the exact (reference heap order) search")
     ap.add_argument("--ef", type=int, default=0, help="fixed ef (skips the sweep)")
     ap.add_argument("--graph-cache", default=None, help="directory to reuse / keep the CPU-built graph")
     ap.add_argument("--no-cpu-search", action="store_true", help="skip the CPU search baseline")
     ap.add_argument("--cpu-repeat", type=int, default=3)
-    ap.add_argument("--cpu-queries", type=int, default=0, help="bound the CPU search sample (0 = all queries)")
-    ap.add_argument("--gt-queries", type=int, default=0, help="queries with ground truth (0 = all)")
+    ap.add_argument("--cpu-queries", type=int, default=None,
+                    help="bound the CPU search sample (0 = all queries; default per workload)")
+    ap.add_argument("--gt-queries", type=int, default=None, help="queries with ground truth (0 = all; default per "
+                                                                 "workload)")
     ap.add_argument("--build", choices=["cpu", "gpu"], default=None,
                     help="graph from the reference's CPU build or the GPU batched build (default per workload)")
     ap.add_argument("--insert-batch", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--precision", choices=["none", "f16", "i8"], default=None,
-                    help="search a low-precision traversal copy + FP32 re-rank (float workloads)")
+    ap.add_argument("--precision", default=None,
+                    help="comma list of low-precision traversal copies (f16, i8) searched + FP32 re-ranked "
+                         "beside the FP32 legs (float32 workloads), or none")
     ap.add_argument("--no-autotune", action="store_true", help="time with the default tuning")
     ap.add_argument("--base-file", default=None, help=".npy / .fbin / .u8bin / .i8bin base vectors instead of "
                                                       "a synthetic workload (with --query-file)")
     ap.add_argument("--query-file", default=None)
     ap.add_argument("--metric", choices=["l2", "ip"], default="l2", help="with --base-file")
     ap.add_argument("--M", type=int, default=32, help="with --base-file")
-    ap.add_argument("--efc", type=int, default=200, help="with --base-file")
+    ap.add_argument("--efc", type=int, default=None, help="efConstruction (with --base-file: 200 by default)")
     ap.add_argument("--serve", action="store_true", help="open-loop serving run through the request loop")
     ap.add_argument("--uneven", action="store_true",
                     help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
     ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--row-limit", type=int, default=None,
+                    help="GPU builds: longest row the reference-rows insertion writes (default per workload; 0: 511)")
     ap.add_argument("--capped-rows", action="store_true",
-                    help="serving build: rows never exceed D ids (the reference's AddEdge can grow a row past D)")
+                    help="GPU builds: rows never exceed D ids (the reference's AddEdge can grow a row past D)")
     ap.add_argument("--insert-share", default="0,0.05",
                     help="comma list of insert shares, one serving run each on the same index")
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
@@ -669,6 +730,12 @@ def main():
     n = args.n or wl["n"]
     nq = args.nq or wl["nq"]
     k = args.k
+    if args.efs is None:
+        args.efs = wl.get("efs", "10,12,14,16,20,24,28,32,40,48,64,96,128,192,256,384,512")
+    if args.gt_queries is None:
+        args.gt_queries = wl.get("gt_queries", 0)
+    if args.cpu_queries is None:
+        args.cpu_queries = wl.get("cpu_queries", 0)
     t0 = time.time()
     if args.base_file:
         from pickle_amd import formats
@@ -676,7 +743,7 @@ def main():
         base = np.ascontiguousarray(formats.read_vectors(args.base_file, 0, args.n))
         queries = np.ascontiguousarray(formats.read_vectors(args.query_file, 0, args.nq))
         n, nq = base.shape[0], queries.shape[0]
-        params = {"metric": args.metric, "M": args.M, "efc": args.efc}
+        params = {"metric": args.metric, "M": args.M, "efc": args.efc or 200}
         wl = dict(wl, desc=f"{os.path.basename(args.base_file)} ({n} x {base.shape[1]} {base.dtype}), "
                            f"{nq} queries from {os.path.basename(args.query_file)}, {args.metric}, M={args.M}, "
                            f"efC={args.efc}", n=n, nq=nq)
@@ -687,6 +754,8 @@ def main():
     dim = base.shape[1]
     metric = params["metric"]
     M, efc = params["M"], params["efc"]
+    if args.efc and not args.base_file:
+        efc = args.efc
     log(f"workload {args.workload}: n={n} nq={nq} dim={dim} {dtype_name} {metric} M={M} efC={efc} "
         f"(generated in {time.time() - t0:.1f}s)")
 
@@ -697,7 +766,9 @@ def main():
                  "reference sources are")
     build = args.build or wl.get("build", "cpu")
     prec = args.precision or wl.get("precision", "none")
-    if prec != "none" and base.dtype != np.float32:
+    if prec == "none":
+        prec = ""
+    if prec and base.dtype != np.float32:
         sys.exit("--precision needs a float32 workload")
 
     work = args.graph_cache or tempfile.mkdtemp(prefix="pk_bench_")
@@ -724,12 +795,15 @@ def main():
         idx.load_file(graph_path)
     else:
         log(f"GPU batched build (insert batches up to {args.insert_batch}) ...")
+        row_limit = args.row_limit if args.row_limit is not None else wl.get("row_limit", 0)
+        idx.set_row_limit(row_limit)
         tb = time.perf_counter()
-        idx.build(base, max_batch=args.insert_batch, seed=args.seed,
+        idx.build(base, max_batch=args.insert_batch, seed=args.seed, exact_rows=not args.capped_rows,
                   progress=lambda d, t: log(f"  inserted {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
         idx.sync()
         build_info = {"build_seconds": time.perf_counter() - tb, "threads": 0, "where": "gpu",
-                      "insert_batch": args.insert_batch}
+                      "insert_batch": args.insert_batch, "capped_rows": bool(args.capped_rows), "efc": efc,
+                      "row_limit": row_limit or 511}
         log(f"GPU build in {build_info['build_seconds']:.1f}s")
         idx.save_file(graph_path)  # for the reference's search on the same graph
         json.dump(build_info, open(meta_path, "w"))
@@ -769,20 +843,29 @@ def main():
     d_dist = abi.DeviceBuffer(nq * k * 4)
     d_cnt = abi.DeviceBuffer(nq * 4)
     d_ctr = abi.DeviceBuffer(nq * abi.COUNTER_DTYPE.itemsize)
-    tidx = None
-    if prec != "none":
+    forms = [x for x in prec.split(",") if x]
+    tidx = {}
+    if forms:
         from pickle_amd import precision
 
-        tp = time.perf_counter()
-        tidx = precision.traversal_index(idx, base, prec)
-        log(f"{prec} traversal copy + FP32 rows in HBM ({time.perf_counter() - tp:.1f}s)")
+        for form in forms:
+            tp = time.perf_counter()
+            tidx[form] = precision.traversal_index(idx, base, form)
+            log(f"{form} traversal copy + FP32 rows in HBM ({time.perf_counter() - tp:.1f}s)")
 
-    def backend(which):
-        if which == "fp32":
-            return idx, (lambda ef, c=False: idx.search_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt,
-                                                               d_ctr if c else None))
-        return tidx, (lambda ef, c=False: tidx.search_rerank_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt,
-                                                                    d_ctr if c else None))
+    # legs: "exact" (k_search, the reference's heap order), "fast" (k_search_fast),
+    # and per precision form "<form>" (fast traversal of the copy + exact FP32 re-rank)
+    def handle(leg):
+        return idx if leg in ("exact", "fast") else tidx[leg]
+
+    def launch(leg, ef, c=False):
+        ctr = d_ctr if c else None
+        if leg == "exact":
+            idx.search_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt, ctr)
+        elif leg == "fast":
+            idx.search_fast_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt, ctr)
+        else:
+            tidx[leg].search_rerank_fast_device(d_q, nq, k, ef, k, d_ids, d_dist, d_cnt, ctr)
 
     vbits = {}
 
@@ -795,146 +878,152 @@ def main():
             cap_bits += 1
         return min(v, cap_bits)
 
-    def sweep(which, target, efs):
-        h, run = backend(which)
+    def sweep(leg, target, efs):
+        h = handle(leg)
         out, chosen = [], None
         for ef in efs:
-            if ef < k:
+            if ef < k or (leg != "exact" and ef > 512):
                 continue
             while True:
-                run(ef)
+                launch(leg, ef)
                 h.sync()
                 h.timer_begin()
-                run(ef)
+                launch(leg, ef)
                 ms = h.timer_end()
                 flags = h.device_error()
-                if flags == 1 and vbits.get(which, default_vbits(ef)) < 20:
+                if leg == "exact" and flags == 1 and vbits.get(leg, default_vbits(ef)) < 20:
                     # a query filled its visited table: a larger one per wave
-                    vbits[which] = max(vbits.get(which, 0), default_vbits(ef)) + 1
-                    h.set_tuning(0, vbits[which], 0)
-                    log(f"[{which}] ef={ef}: visited table full; retrying with 2^{vbits[which]} slots per wave")
+                    vbits[leg] = max(vbits.get(leg, 0), default_vbits(ef)) + 1
+                    h.set_tuning(0, vbits[leg], 0)
+                    log(f"[{leg}] ef={ef}: visited table full; retrying with 2^{vbits[leg]} slots per wave")
                     continue
                 break
             if flags:
-                sys.exit(f"[{which}] ef={ef}: device reported search failure flags {flags} "
+                sys.exit(f"[{leg}] ef={ef}: device reported search failure flags {flags} "
                          "(1 visited table full, 2 bad arguments, 4 candidate heap full)")
             ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
             r = recall_at_k(ids, gt_ids, k)
             out.append([ef, round(r, 4), round(nq / (ms / 1e3), 1)])
-            log(f"[{which}] ef={ef:4d} recall@{k}={r:.4f} qps={nq / (ms / 1e3):,.0f}")
+            log(f"[{leg}] ef={ef:4d} recall@{k}={r:.4f} qps={nq / (ms / 1e3):,.0f}")
             if r >= target and chosen is None:
                 chosen = ef
                 if not args.ef:
                     break
         if chosen is None:
             chosen = out[-1][0]
-            log(f"[{which}] no ef reached recall {target}; using ef={chosen}")
+            log(f"[{leg}] no ef reached recall {target}; using ef={chosen}")
         return chosen, [x[1] for x in out if x[0] == chosen][0], out
 
     tuned = {}
 
-    def autotune(which, ef):
-        """The library's autotuner on the first 4,096 queries at this ef (its
-        choice does not change results); skipped with --no-autotune."""
+    def autotune(leg, ef):
+        """The library's autotuner at this ef on up to 16,384 of the queries
+        (the exact search's tuning cannot change results; the fast mode's
+        width can, and the sweep after it measures the recall it gives)."""
         if args.no_autotune:
             return
-        h, _ = backend(which)
         ns = min(nq, 16384)
-        if which == "fp32":
-            dq, hq = d_q, None
+        if leg == "exact":
+            t = idx.autotune(d_q, ns, k, ef)
         else:
-            qc = queries[:ns]
-            if prec == "f16":
-                qc = qc.astype(np.float16)
-            else:
-                from pickle_amd import precision as P
-                off, sc = P.int8_quantizer(base, mcode)
-                inv = (np.float32(1.0) / sc).astype(np.float32)
-                qc = np.clip(np.rint(((qc - off).astype(np.float32) * inv).astype(np.float32)), -127, 127)
-                qc = qc.astype(np.int8)
-            hq = abi.DeviceBuffer.from_array(np.ascontiguousarray(qc))
-            dq = hq
-        t = h.autotune(dq, ns, k, ef)
-        # the tuner times a sample of the batch with k outputs; keep its choice
-        # only if the whole batch, as this leg runs it, is faster with it
-        _, run = backend(which)
+            t = idx.autotune_fast(d_q, ns, k, min(ef, 512))
+            for h in tidx.values():
+                h.set_fast_tuning(t["width"], t["hash_bits"], t["waves_per_group"], t["blocks_per_cu"])
+        tuned[leg] = t
+        log(f"[{leg}] autotuned at ef={ef}: {t}")
 
-        def batch_ms():
-            run(ef)
-            h.sync()
-            h.timer_begin()
-            run(ef)
-            return h.timer_end()
-
-        tuned_ms = batch_ms()
-        h.set_tuning(0, vbits.get(which, 0), 0)
-        base_ms = batch_ms()
-        if tuned_ms < base_ms:
-            h.set_tuning(t["ccap"], t["vbits"], t["blocks_per_cu"])
-            tuned[which] = dict(t, kept=True)
-        else:
-            tuned[which] = dict(t, kept=False)
-        log(f"[{which}] autotuned at ef={ef}: {t}; whole batch {tuned_ms:.2f} ms tuned vs {base_ms:.2f} ms "
-            f"default -> {'tuned' if tuned_ms < base_ms else 'default'}")
-
-    def timed(which, ef):
-        h, run = backend(which)
+    def timed(leg, ef, steps):
+        h = handle(leg)
         for _ in range(args.warmup):
-            run(ef)
+            launch(leg, ef)
         h.sync()
         tw0 = time.perf_counter()
         h.timer_begin()
-        for _ in range(args.steps):
-            run(ef)
+        for _ in range(steps):
+            launch(leg, ef)
         ev_ms = h.timer_end()
         h.sync()
-        return time.perf_counter() - tw0, ev_ms / 1e3 / args.steps
+        if h.device_error():
+            sys.exit(f"[{leg}] device reported a search failure")
+        return time.perf_counter() - tw0, ev_ms / 1e3 / steps
+
+    def counters(leg, ef):
+        launch(leg, ef, True)
+        handle(leg).sync()
+        return d_ctr.to_array(abi.COUNTER_DTYPE, (nq,))
+
+    def alg_bytes_of(ctr, elem, rerank_ef=0):
+        """Algorithmic bytes of a launch in the reference Profiler's units: 4 B
+        per adjacency entry scanned, 8 B of node metadata per hop, one row per
+        distance, the query; the precision forms add the FP32 rows of the ef
+        re-ranked candidates and the FP32 query."""
+        hops = ctr["hops"].sum(axis=1).astype(np.int64)
+        nbr = ctr["neighbors"].sum(axis=1).astype(np.int64)
+        dists = ctr["dists"].sum(axis=1).astype(np.int64)
+        b = int((4 * nbr + 8 * hops + elem * dim * dists).sum() + nq * elem * dim)
+        if rerank_ef:
+            b += nq * (rerank_ef * (4 * dim + 4) + 4 * dim)
+        return b, hops, nbr, dists
 
     efs = [args.ef] if args.ef else [int(e) for e in args.efs.split(",")]
-    comparison = None
-    if prec == "none":
-        main_be = "fp32"
-        chosen, recall, sweep_out = sweep("fp32", args.recall_target, efs)
-    else:
-        ef32, r32, sw32 = sweep("fp32", args.recall_target, efs)
-        autotune("fp32", ef32)
-        wall32, ks32 = timed("fp32", ef32)
-        qps32 = nq * args.steps / wall32
-        # the precision form at equal recall: the first ef within 0.002 of the FP32 recall
-        main_be = "rerank"
-        chosen, recall, sweep_out = sweep("rerank", r32 - 0.002, efs)
-        comparison = {"fp32_traversal": {"ef": ef32, "recall@10": r32, "qps": round(qps32, 1),
-                                         "kernel_ms": round(ks32 * 1e3, 4), "sweep_ef_recall_qps": sw32}}
-    h_main, run_main = backend(main_be)
-    if h_main.device_error():
-        sys.exit("device reported a search failure")
+    mode = args.mode
+    results = {}
+    # the exact search: the reference's heap order, the CPU baseline's ef
+    ef_x, r_x, sw_x = sweep("exact", args.recall_target, efs)
+    if mode == "exact":
+        autotune("exact", ef_x)
+    steps_x = args.steps if mode == "exact" else max(3, args.steps // 4)
+    wall_x, ks_x = timed("exact", ef_x, steps_x)
+    results["exact"] = {"ef": ef_x, "recall@10": r_x, "qps": round(nq * steps_x / wall_x, 1),
+                        "kernel_ms": round(ks_x * 1e3, 4), "sweep_ef_recall_qps": sw_x, "kernel": "k_search"}
+    exact_ids = d_ids.to_array(np.int32, (nq, k))
+    if mode == "fast":
+        autotune("fast", ef_x)
+        ef_f, r_f, sw_f = sweep("fast", args.recall_target, efs)
+        # a first timing for the precision forms' ratio; the headline is timed last
+        steps_f = max(3, args.steps // 4)
+        wall_f, ks_f = timed("fast", ef_f, steps_f)
+        results["fast"] = {"ef": ef_f, "recall@10": r_f, "qps": round(nq * steps_f / wall_f, 1),
+                           "kernel_ms": round(ks_f * 1e3, 4), "sweep_ef_recall_qps": sw_f, "kernel": "k_search_fast"}
+    main_leg = mode
+    r_main = results[main_leg]["recall@10"]
+    for form in forms:
+        # the precision form at equal recall: the first ef within 0.002 of the FP32 leg's recall
+        ef_p, r_p, sw_p = sweep(form, r_main - 0.002, efs)
+        steps_p = max(3, args.steps // 4)
+        wall_p, ks_p = timed(form, ef_p, steps_p)
+        results[form] = {"ef": ef_p, "recall@10": r_p, "qps": round(nq * steps_p / wall_p, 1),
+                         "kernel_ms": round(ks_p * 1e3, 4), "sweep_ef_recall_qps": sw_p,
+                         "kernel": "k_quantize + k_search_fast + k_rerank",
+                         "speedup_vs_fp32": round(nq * steps_p / wall_p / results[main_leg]["qps"], 3)}
+        log(f"{form} traversal + FP32 re-rank: {results[form]['qps']:,.0f} QPS at recall {r_p:.4f} (ef {ef_p}) vs "
+            f"FP32 {main_leg} {results[main_leg]['qps']:,.0f} QPS at {r_main:.4f} -> "
+            f"{results[form]['speedup_vs_fp32']:.3f}x")
 
-    # work counters (untimed pass, as the reference's second profiler pass)
-    run_main(chosen, True)
-    h_main.sync()
-    ctr = d_ctr.to_array(abi.COUNTER_DTYPE, (nq,))
-    elem = {"none": base.dtype.itemsize, "f16": 2, "i8": 1}[prec]
-    hops = ctr["hops"].sum(axis=1).astype(np.int64)
-    nbr = ctr["neighbors"].sum(axis=1).astype(np.int64)
-    dists = ctr["dists"].sum(axis=1).astype(np.int64)
-    # algorithmic bytes (reference Profiler units): 4 B per adjacency entry
-    # scanned, 8 B of node metadata per hop, one row per distance, the query;
-    # the precision form adds the FP32 rows of the ef re-ranked candidates
-    alg_bytes = int((4 * nbr + 8 * hops + elem * dim * dists).sum() + nq * elem * dim)
-    if prec != "none":
-        alg_bytes += nq * (chosen * (4 * dim + 4) + 4 * dim)
-    khz = abi.wall_clock_khz()
-    lat_ms = ctr["latency_ticks"].astype(np.float64) / khz
-
-    autotune("fp32" if main_be == "fp32" else "rerank", chosen)
-    wall, kernel_s = timed(main_be, chosen)
+    # the headline: the main leg, timed again last so its launches are the
+    # last `steps` of its kernel in a profile
+    ef_m = results[main_leg]["ef"]
+    ctr_m = counters(main_leg, ef_m)
+    wall, kernel_s = timed(main_leg, ef_m, args.steps)
     qps = nq * args.steps / wall
+    results[main_leg]["qps"] = round(qps, 1)
+    results[main_leg]["kernel_ms"] = round(kernel_s * 1e3, 4)
+    for form in forms:
+        results[form]["speedup_vs_fp32"] = round(results[form]["qps"] / qps, 3)
+    elem = base.dtype.itemsize
+    counted, hops, nbr, dists = alg_bytes_of(ctr_m, elem)
+    # algorithmic bytes: the reference algorithm's work at this ef (the exact
+    # search's counters); the fast kernel's own counters also include the
+    # rows its LDS filter made it read twice
+    ctr_ref = counters("exact", ef_m) if main_leg == "fast" else ctr_m
+    alg_bytes, hops_r, nbr_r, dists_r = alg_bytes_of(ctr_ref, elem)
     achieved = alg_bytes / kernel_s / 1e9
-    grid, lds = h_main.last_launch()
-    log(f"timed: {args.steps} x {nq} queries at ef={chosen}: {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
+    khz = abi.wall_clock_khz()
+    lat_ms = ctr_m["latency_ticks"].astype(np.float64) / khz
+    grid, lds = idx.last_launch()
+    log(f"timed: {args.steps} x {nq} queries at ef={ef_m} ({main_leg}): {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
-    # per-result distance error against FP64 sums of the same (query, row)
-    # pairs, on the ground-truth queries (the timed launches' outputs)
+    # the main leg's results again (the timed launches' outputs)
     r_ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
     r_d = d_dist.to_array(np.float32, (nq, k))[:gt_nq].astype(np.float64)
     ok = r_ids >= 0
@@ -945,11 +1034,6 @@ def main():
     dist_err = {"max_rel": float(rel[ok].max()), "mean_rel": float(rel[ok].mean()), "queries": int(gt_nq)}
     log(f"returned distances vs FP64: max relative error {dist_err['max_rel']:.3g}, "
         f"mean {dist_err['mean_rel']:.3g}")
-    if comparison:
-        comparison["speedup_vs_fp32_traversal"] = round(qps / comparison["fp32_traversal"]["qps"], 3)
-        log(f"{prec} traversal + FP32 re-rank: {qps:,.0f} QPS at recall {recall:.4f} vs FP32 traversal "
-            f"{comparison['fp32_traversal']['qps']:,.0f} QPS at {comparison['fp32_traversal']['recall@10']:.4f}"
-            f" -> {comparison['speedup_vs_fp32_traversal']:.3f}x")
 
     # known-size copy (k_copy16, 256 MiB read + 256 MiB written, twice):
     # the kernel against which profiled HBM byte counters are calibrated
@@ -962,10 +1046,11 @@ def main():
     calib2.free()
 
     # cpu_baseline leg: the reference's AnnSearch on the host cores, same
-    # graph, FP32 data, the FP32 operating point
+    # graph and data, at the exact search's ef (the reference's own operating
+    # point at the recall target)
     cpu = None
+    hostcpu = cpu_info()
     if not args.no_cpu_search:
-        cpu_ef = comparison["fp32_traversal"]["ef"] if comparison else chosen
         cnq = min(nq, args.cpu_queries) if args.cpu_queries else nq
         if cnq < nq:
             queries[:cnq].tofile(q_path + ".sample")
@@ -973,55 +1058,56 @@ def main():
         else:
             cq_path = q_path
         out = os.path.join(work, "cpu_res")
-        st = cpu_search(binary, graph_path, base_path, cq_path, n, cnq, dim, dtype_name, k, cpu_ef, threads,
+        st = cpu_search(binary, graph_path, base_path, cq_path, n, cnq, dim, dtype_name, k, ef_x, threads,
                         args.cpu_repeat, out)
-        cpu_ids = read_ref_results(f"{out}.ef{cpu_ef}", cnq, k)
+        cpu_ids = read_ref_results(f"{out}.ef{ef_x}", cnq, k)
         r_n = min(cnq, gt_nq)
         cpu_recall = recall_at_k(cpu_ids[:r_n], gt_ids[:r_n], k)
-        entry = {"value": round(st["qps"], 1), "unit": "queries/s", "cores": threads, "kind": "reference",
-                 "sample": f"{cnq} queries, ef={cpu_ef}, k={k}, best of {args.cpu_repeat} runs, reference "
-                           f"AnnSearch compiled from source ({isa} distance path) on {cpu_model()}, "
-                           f"{'reference-built' if build == 'cpu' else 'GPU-built'} graph",
-                 "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
-                 "build_seconds": round(build_info["build_seconds"], 2),
-                 "build_threads": build_info["threads"], "build_where": build_info.get("where", "cpu")}
-        if not comparison:
-            same = float(np.mean(np.all(cpu_ids == d_ids.to_array(np.int32, (nq, k))[:cnq], axis=1)))
-            entry["ids_identical_to_gpu"] = round(same, 4)
-            log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads, recall {cpu_recall:.4f}, "
-                f"GPU ids identical on {same * 100:.2f}% of queries")
-        else:
-            log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads at ef={cpu_ef}, recall {cpu_recall:.4f}")
-        cpu = entry
+        same = float(np.mean(np.all(cpu_ids == exact_ids[:cnq], axis=1)))
+        cpu = {"value": round(st["qps"], 1), "unit": "queries/s", "cores": threads, "kind": "reference",
+               "sample": f"{cnq} of the {nq} queries, ef={ef_x}, k={k}, best of {args.cpu_repeat} runs, reference "
+                         f"AnnSearch compiled from source ({isa} distance path), OpenMP on {threads} threads "
+                         f"(every core this process may use) of {hostcpu['model']}, "
+                         f"{'reference-built' if build == 'cpu' else 'GPU-built'} graph",
+               "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
+               "ids_identical_to_gpu_exact": round(same, 4), "host": hostcpu,
+               "build_seconds": round(build_info["build_seconds"], 2),
+               "build_threads": build_info["threads"], "build_where": build_info.get("where", "cpu")}
+        log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads at ef={ef_x}, recall {cpu_recall:.4f}, "
+            f"GPU exact ids identical on {same * 100:.2f}% of queries; GPU {main_leg} = "
+            f"{qps / st['qps']:.1f}x")
 
-    traffic, traffic_src, l2_hit = (profiled_traffic(args.workload, n, nq, k, chosen) if prec == "none"
-                                    else (None, None, None))
-    search_desc = "exact (reference heap order)" if prec == "none" else \
-        f"{prec} traversal copy + exact FP32 re-rank of the ef candidates"
-    dtype_out = dtype_name if prec == "none" else f"{prec} traversal, f32 re-rank"
+    traffic, traffic_src, l2_hit = profiled_traffic(args.workload, n, nq, k, ef_m, main_leg)
     cfg = {"workload": wl["desc"] if (n, nq) == (wl["n"], wl["nq"]) else f"{wl['desc']} (n={n}, nq={nq})",
-           "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc, "k": k, "ef": chosen, "recall@10": recall,
+           "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc, "k": k, "ef": ef_m, "recall@10": r_main,
            "recall_queries": gt_nq, "graph": "reference CPU build" if build == "cpu" else
-           f"GPU batched build ({build_info['build_seconds']:.1f}s)", "search": search_desc,
+           f"GPU batched build ({build_info['build_seconds']:.1f}s, efC {build_info.get('efc', efc)}, "
+           + ("rows capped at D" if build_info.get('capped_rows') else
+              f"reference rows up to {build_info.get('row_limit', 511)} ids") + ")",
+           "search": {"exact": "exact (reference heap order)",
+                      "fast": "fast mode (sorted register beam, width-w expansion, LDS visited filter)"}[main_leg],
            "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
            "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
            "distance_error_vs_fp64": dist_err,
-           "batch_latency_ms": round(wall / args.steps * 1e3, 4), "sweep_ef_recall_qps": sweep_out}
-    if comparison:
-        cfg["precision_comparison"] = comparison
+           "batch_latency_ms": round(wall / args.steps * 1e3, 4), "legs": results}
     if tuned:
         cfg["autotuned"] = tuned
     line = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": dtype_out, "data": "synthetic", "config": cfg,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype_name, "data": "synthetic", "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src, "l2_hit_rate": l2_hit,
-                     "kernel": "k_search" if prec == "none" else "k_quantize + k_search + k_rerank",
-                     "kernel_ms": round(kernel_s * 1e3, 4), "algorithmic_bytes_per_launch": alg_bytes,
-                     "per_query": {"hops": round(float(hops.mean()), 2), "neighbors": round(float(nbr.mean()), 2),
-                                   "dists": round(float(dists.mean()), 2)}},
+                     "kernel": results[main_leg]["kernel"], "kernel_ms": round(kernel_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "algorithmic_per_query": {"hops": round(float(hops_r.mean()), 2),
+                                               "neighbors": round(float(nbr_r.mean()), 2),
+                                               "dists": round(float(dists_r.mean()), 2)},
+                     "kernel_counted_bytes_per_launch": counted,
+                     "kernel_per_query": {"hops": round(float(hops.mean()), 2),
+                                          "neighbors": round(float(nbr.mean()), 2),
+                                          "dists": round(float(dists.mean()), 2)}},
         "cpu_baseline": cpu,
         "ground_truth": ground_truth,
     }

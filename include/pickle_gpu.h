@@ -178,6 +178,15 @@ int pk_search_fast_device(pk_index *h, const void *d_queries, int64_t nq, const 
  * tuning changes a result except width. */
 int pk_set_fast_tuning(pk_index *h, int width, int hash_bits, int waves_per_group, int blocks_per_cu);
 
+/* Autotuner of the fast mode (no counterpart in the reference): coordinate
+ * descent at one (k, ef) on the caller's device queries over queries per
+ * workgroup, the LDS filter size, the expansion width and resident
+ * workgroups per CU; keeps the fastest setting on the handle.  out_tuning
+ * (4 ints: width, hash_bits, waves_per_group, blocks_per_cu) and out_ms may
+ * be NULL.  Width changes results (recall rises with it); the rest do not. */
+int pk_autotune_fast(pk_index *h, const void *d_queries, int64_t nq, int32_t k, int32_t ef,
+                     int32_t *out_tuning, float *out_ms);
+
 /* pk_search_device_ex without allocation.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
@@ -315,6 +324,13 @@ int pk_index_init_build(pk_index *h, const uint8_t *levels);
  * handles). */
 int pk_index_insert(pk_index *h, const int32_t *ext_ids, int64_t n, int max_batch, int flags);
 
+/* Longest row a PK_BUILD_EXACT_ROWS insertion writes: the reference's rows
+ * grow without bound when its heuristic re-run keeps more than D ids
+ * (hnsw_layer.hpp:441-447); here at most `limit` ids (0: 511, else 2M..511)
+ * are kept and a neighbour that would land past them is dropped, as a full
+ * row's occluded neighbour is.  No counterpart in the reference. */
+int pk_set_build_row_limit(pk_index *h, int limit);
+
 /* ---- precision form: low-precision traversal copy + exact FP32 re-rank ----
  * No counterpart in the reference (its search reads one copy of the data).
  * An index of dtype PK_FLOAT16 or PK_INT8 holds the traversal copy; these
@@ -343,6 +359,13 @@ int pk_index_copy_graph(pk_index *dst, pk_index *src);
 int pk_search_rerank_device(pk_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
                             int32_t k_stride, int32_t *d_out_ids, float *d_out_dists, int32_t *d_out_count,
                             pk_query_counters *d_counters);
+
+/* pk_search_rerank_device with the fast-mode traversal (pk_search_fast_device
+ * and its tuning) on the low-precision copy; ef <= 512.  The re-rank is the
+ * same exact FP32 one. */
+int pk_search_rerank_fast_device(pk_index *h, const float *d_queries, int64_t nq, int32_t k, int32_t ef,
+                                 int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
+                                 int32_t *d_out_count, pk_query_counters *d_counters);
 /* The same with host buffers (copies in, runs, copies out).  As in
  * pk_search, queries whose traversal fills its visited table run again with
  * larger tables (up to 2^20 slots); PK_ERR_DEVICE only if that still fails,

@@ -27,7 +27,8 @@ EXPORTS = [
     "pk_last_error", "pk_version", "pk_index_create", "pk_index_destroy", "pk_index_set_base",
     "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
-    "pk_search_device_static", "pk_search_fast_device", "pk_set_fast_tuning",
+    "pk_search_device_static", "pk_search_fast_device", "pk_set_fast_tuning", "pk_search_rerank_fast_device",
+    "pk_autotune_fast", "pk_set_build_row_limit",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
     "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
@@ -77,6 +78,9 @@ def lib():
             "pk_search_device_static": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_fast_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_set_fast_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "pk_search_rerank_fast_device": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_autotune_fast": (C.c_int, [vp, vp, i64, i32, i32, vp, vp]),
+            "pk_set_build_row_limit": (C.c_int, [vp, C.c_int]),
             "pk_device_error": (C.c_int, [vp, vp]),
             "pk_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "pk_last_launch": (C.c_int, [vp, vp, vp]),
@@ -397,6 +401,20 @@ class Index:
                         blocks_per_cu: int = 0) -> None:
         check(lib().pk_set_fast_tuning(self.h, width, hash_bits, waves_per_group, blocks_per_cu))
 
+    def autotune_fast(self, d_queries: DeviceBuffer, nq: int, k: int, ef: int) -> dict:
+        out = (C.c_int32 * 4)()
+        ms = C.c_float()
+        check(lib().pk_autotune_fast(self.h, d_queries.ptr, nq, k, ef, out, C.byref(ms)))
+        return {"width": out[0], "hash_bits": out[1], "waves_per_group": out[2], "blocks_per_cu": out[3],
+                "ms": ms.value}
+
+    def search_rerank_fast_device(self, d_queries: "DeviceBuffer", nq: int, k: int, ef: int, k_stride: int,
+                                  d_ids: "DeviceBuffer", d_dists: "DeviceBuffer", d_count: "DeviceBuffer" = None,
+                                  d_counters: "DeviceBuffer" = None) -> None:
+        check(lib().pk_search_rerank_fast_device(self.h, d_queries.ptr, nq, k, ef, k_stride, d_ids.ptr, d_dists.ptr,
+                                                 d_count.ptr if d_count else None,
+                                                 d_counters.ptr if d_counters else None))
+
     def search_fast(self, queries: np.ndarray, k, ef, k_stride: int | None = None, counters: bool = False):
         """Fast-mode search of host queries: (ids, dists, counts[, counters]).
         k / ef may be ints or per-query arrays."""
@@ -429,6 +447,10 @@ class Index:
         if counters:
             return ids, ds, cnt, ctr
         return ids, ds, cnt
+
+    def set_row_limit(self, limit: int) -> None:
+        """Longest row a reference-rows build writes (0: 511)."""
+        check(lib().pk_set_build_row_limit(self.h, limit))
 
     def init_build(self, levels: np.ndarray) -> None:
         """InitBuffer: levels for every external id of the capacity."""

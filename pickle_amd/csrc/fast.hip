@@ -48,6 +48,7 @@ namespace pk {
 
 constexpr uint64_t kKeyInf = ~0ull;
 constexpr int kFastCap = 256;  // new neighbours gathered before a flush (distances + merge)
+constexpr int kInsertMax = 6;  // accepted neighbours inserted one by one rather than sorted and merged
 
 struct FastArgs {
     const LayerDev* layers;
@@ -102,6 +103,17 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
     return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
 }
+// lane i gets lane i - 1's key, lane 0 gets `first`: a DPP wave shift
+// right by one (wave_shr:1, a VALU move on CDNA), the vacated lane 0 keeping
+// the old value supplied
+__device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint64_t first) {
+    constexpr int kWaveShr1 = 0x138;
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)first, (int)(uint32_t)v, kWaveShr1, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(first >> 32), (int)(uint32_t)(v >> 32), kWaveShr1, 0xf,
+                                               0xf, false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
 // lane l's key, uniform (v_readlane reads any lane, whatever its exec bit)
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -154,6 +166,7 @@ struct Beam {
     // duplicate dropped from W is replaced from there.
     __device__ __forceinline__ uint64_t merge(uint64_t s, int lane) {
         const uint64_t srev = shfl64(s, kWave - 1 - lane);
+        if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
         const uint64_t last = key[RR - 1];
         const uint64_t t = min64(last, srev);
         const uint64_t over = bitonic_merge64(max64(last, srev), lane, true);
@@ -170,9 +183,15 @@ struct Beam {
                     const uint64_t a = key[r], b = key[r + J];
                     key[r] = min64(a, b);
                     key[r + J] = max64(a, b);
+                    if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
                 }
+        // one row at a time: interleaving the rows' exchanges raises register
+        // pressure enough to cost occupancy with eight rows
 #pragma unroll
-        for (int r = 0; r < RR; r++) key[r] = bitonic_merge64(key[r], lane, true);
+        for (int r = 0; r < RR; r++) {
+            key[r] = bitonic_merge64(key[r], lane, true);
+            if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
+        }
         return over;
     }
 
@@ -191,6 +210,7 @@ struct Beam {
         int total = 0;
 #pragma unroll
         for (int r = 0; r <= RR; r++) {
+            if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
             const uint64_t row_before = r > 0 ? readlane64(v[r - 1], kWave - 1) : kKeyInf;
             uint64_t prev = shfl64(v[r], lane > 0 ? lane - 1 : 0);
             if (lane == 0) prev = row_before;
@@ -215,6 +235,30 @@ struct Beam {
         for (int r = 0; r < RR; r++) key[r] = scratch[64 * r + lane];
         __builtin_amdgcn_wave_barrier();
         return total;
+    }
+
+    // Inserts one pending key c at its sorted position, moving the keys
+    // after it one place on (the key at position ef - 1 leaves), unless the
+    // beam holds the same (distance, id) already or c sorts at or past ef.
+    __device__ __forceinline__ void insert1(uint64_t c, int ef, int lane) {
+        int pos = 0;
+        bool dup = false;
+#pragma unroll
+        for (int r = 0; r < RR; r++) {
+            pos += __builtin_popcountll(ballot(key[r] < c));
+            dup |= ballot(key[r] != kKeyInf && (key[r] >> 1) == (c >> 1)) != 0;
+        }
+        if (dup || pos >= ef) return;
+        uint64_t carry = kKeyInf;  // last key of the row before
+#pragma unroll
+        for (int r = 0; r < RR; r++) {
+            const uint64_t last = readlane64(key[r], kWave - 1);
+            const uint64_t up = shift_up1(key[r], carry);
+            const int p = 64 * r + lane;
+            key[r] = p < pos ? key[r] : (p == pos ? c : up);
+            carry = last;
+            if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
+        }
     }
 
     // positions >= ef leave the beam
@@ -304,8 +348,27 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
             const uint32_t worst = (uint32_t)(W.at(ef - 1) >> 32);  // hnsw.hpp:425: top > dist
             if (!((uint32_t)(key >> 32) < worst)) key = kKeyInf;
         }
-        if (ballot(key != kKeyInf) == 0) continue;
-        const uint64_t over = W.merge(bitonic_sort64(key, lane), lane);
+        uint64_t pass = ballot(key != kKeyInf);
+        if (pass == 0) continue;
+        // (not with eight rows: the second path there costs the kernel a
+        // quarter of its occupancy in registers)
+        if (RR < 8 && __builtin_popcountll(pass) <= kInsertMax) {
+            // few accepted: one at a time, each against the beam as it stands
+            while (pass) {
+                const int ln = __builtin_ctzll(pass);
+                pass &= pass - 1;
+                const uint64_t kc = readlane64(key, ln);
+                if (len >= ef && !((uint32_t)(kc >> 32) < (uint32_t)(W.at(ef - 1) >> 32))) continue;
+                W.insert1(kc, ef, lane);
+                W.cap(ef, lane);  // the key shifted to position ef leaves
+                len = W.count();
+            }
+            continue;
+        }
+        const uint64_t sorted = bitonic_sort64(key, lane);
+        if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
+        const uint64_t over = W.merge(sorted, lane);
+        if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
         W.dedup(over, scratch, lane);
         W.cap(ef, lane);
         len = W.count();
@@ -570,7 +633,7 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     const FastTuning& t = h->fast;
     const int rr = beam_rows(ef_max);
     const int width = t.width > 0 ? t.width : 2;
-    const int hb = t.hash_bits > 0 ? t.hash_bits : 12;
+    const int hb = t.hash_bits > 0 ? t.hash_bits : 11;
     const int wpg = t.waves_per_group > 0 ? t.waves_per_group : 2;
     size_t per_wave = ((size_t)4 << hb) + (size_t)kFastCap * 8 + (size_t)64 * (rr + 1) * 8 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
@@ -610,6 +673,93 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("fast search launch: ") + hipGetErrorString(e));
     h->last_grid = grid;
     h->last_lds = lds;
+    return PK_OK;
+}
+
+// Coordinate descent over the fast mode's tuning at one (k, ef) on the
+// caller's queries: waves per workgroup (queries per workgroup), the LDS
+// filter size, the expansion width, then resident workgroups per CU; each
+// setting timed over two launches after one warm-up.  Keeps the fastest
+// (width may change results: recall at a fixed ef rises with it).
+int pk_autotune_fast(pk_index* h, const void* d_queries, int64_t nq, int32_t k, int32_t ef, int32_t* out_tuning,
+                     float* out_ms) {
+    if (!h || !d_queries || nq < 1 || k < 1 || ef < k || ef > 512) return set_error(PK_ERR_ARG, "bad arguments");
+    if (h->servers_alive) return set_error(PK_ERR_STATE, "destroy the serving loops before tuning");
+    void* buf = nullptr;
+    const size_t out_bytes = (size_t)nq * k * 8 + (size_t)nq * 4;
+    if (hipMalloc(&buf, out_bytes) != hipSuccess) return set_error(PK_ERR_NOMEM, "tuner output buffers");
+    int32_t* ids = static_cast<int32_t*>(buf);
+    float* ds = reinterpret_cast<float*>(static_cast<uint8_t*>(buf) + (size_t)nq * k * 4);
+    int32_t* cnt = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(buf) + (size_t)nq * k * 8);
+    const FastTuning saved = h->fast;
+    auto time_one = [&](const FastTuning& t, float* ms) -> int {
+        h->fast = t;
+        int r = pk_search_fast_device(h, d_queries, nq, nullptr, nullptr, k, ef, ef, k, ids, ds, cnt, nullptr);
+        if (r) return r;
+        if (hipEventRecord(h->ev0, h->stream) != hipSuccess) return set_error(PK_ERR_HIP, "event record");
+        for (int i = 0; i < 2 && !r; i++)
+            r = pk_search_fast_device(h, d_queries, nq, nullptr, nullptr, k, ef, ef, k, ids, ds, cnt, nullptr);
+        if (r) return r;
+        if (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
+            hipEventElapsedTime(ms, h->ev0, h->ev1) != hipSuccess)
+            return set_error(PK_ERR_HIP, "event timing");
+        *ms /= 2;
+        int32_t flags = 0;
+        if (hipMemcpy(&flags, h->sc.err, sizeof(flags), hipMemcpyDeviceToHost) != hipSuccess)
+            return set_error(PK_ERR_HIP, "flags read");
+        if (flags) return set_error(PK_ERR_DEVICE, "fast search failed on the device (flags " + std::to_string(flags) + ")");
+        return PK_OK;
+    };
+    FastTuning best = saved;
+    float best_ms = 0;
+    int rc = time_one(best, &best_ms);
+    if (rc) {
+        h->fast = saved;
+        (void)hipFree(buf);
+        return rc;
+    }
+    auto try_set = [&](FastTuning t) {
+        float ms = 0;
+        if (time_one(t, &ms) == PK_OK && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    for (int w : {1, 2, 4}) {
+        FastTuning t = best;
+        t.waves_per_group = w;
+        try_set(t);
+    }
+    for (int b : {9, 10, 11, 12, 13}) {
+        FastTuning t = best;
+        t.hash_bits = b;
+        try_set(t);
+    }
+    for (int w : {1, 2, 3, 4}) {
+        FastTuning t = best;
+        t.width = w;
+        try_set(t);
+    }
+    {
+        // resident workgroups per CU: up to three below what fits
+        h->fast = best;
+        const int last = h->last_grid;
+        const int max_bpc = std::max(1, (last + h->num_cus - 1) / h->num_cus);
+        for (int b = max_bpc - 1; b >= std::max(1, max_bpc - 3); b--) {
+            FastTuning t = best;
+            t.blocks_per_cu = b;
+            try_set(t);
+        }
+    }
+    h->fast = best;
+    (void)hipFree(buf);
+    if (out_tuning) {
+        out_tuning[0] = best.width;
+        out_tuning[1] = best.hash_bits;
+        out_tuning[2] = best.waves_per_group;
+        out_tuning[3] = best.blocks_per_cu;
+    }
+    if (out_ms) *out_ms = best_ms;
     return PK_OK;
 }
 

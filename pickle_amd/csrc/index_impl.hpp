@@ -137,6 +137,7 @@ struct pk_index {
     // search scratch of the handle's own stream
     pk::SearchCtx sc;
     int cg_cap = 16384;
+    int row_limit = 0;  // reference-rows builds: longest row AddEdge writes (0 = 511)
     pk::SearchTuning tuning;
     pk::FastTuning fast;
     pk::IoBuf io[8];

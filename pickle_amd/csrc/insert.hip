@@ -78,6 +78,7 @@ struct InsertArgs {
     int32_t cg_cap, vg_bits;
     int32_t wcap, ccap, lds_per_wave;
     int32_t exact_rows;
+    int32_t row_limit;       // longest row AddEdge writes: D (capped rows) up to kRowMax - 1
     int32_t row_cache;       // AddEdge: rows short enough to stage whole in LDS (< row_cache ids)
     int32_t row_cache_long;  // AddEdge on longer rows: kept-set vectors held in LDS
     int64_t rows_after[PK_MAX_LEVELS];  // layer sizes once the batch is in
@@ -490,7 +491,7 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
                 B.deg[vid] = dg + (dg < B.D);
             }
         }
-    } else if ((!A.exact_rows && offset >= B.D) || offset >= kRowMax - 1) {
+    } else if (offset >= (A.exact_rows ? A.row_limit : B.D)) {
         // a full row of closer heuristic neighbours: the reference writes
         // slot D (hnsw_layer.hpp:420, 441-447); rows capped at D drop the
         // edge, and so do reference rows at their limit of kRowMax - 1 ids
@@ -544,7 +545,7 @@ __device__ void add_edge(const InsertArgs& A, const LayerBuild& B, int64_t n_row
             DF dv;
             dv.load_query(cache + (size_t)v1_slot * vs, g, qdim);
             const bool occl = kept_closer(dv, cache, B.vec, vs, nb_id, nb_slot, in_lds, cur_h, entry_dist(v1), lane);
-            const bool keep = !occl && (A.exact_rows || cur_h < B.D) && cur_h < kRowMax - 1;
+            const bool keep = !occl && cur_h < (A.exact_rows ? A.row_limit : B.D);
             if (keep) {
                 int slot = v1_slot;
                 if (!cached) {
@@ -958,6 +959,7 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
         a.ccap = ccap;
         a.lds_per_wave = (int)lds_a;
         a.exact_rows = (flags & PK_BUILD_EXACT_ROWS) != 0;
+        a.row_limit = h->row_limit > 0 ? std::min(h->row_limit, kRowMax - 1) : kRowMax - 1;
         a.row_cache = row_cache;
         a.row_cache_long = row_cache_long;
         for (int l = 0; l < PK_MAX_LEVELS; l++) a.rows_after[l] = h->layers[l].n;
@@ -1006,5 +1008,13 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
     ICHK(hipMemcpy(&err, h->ins.err, 4, hipMemcpyDeviceToHost));
     if (err) return set_error(PK_ERR_DEVICE, "insertion failed on the device (flags " + std::to_string(err) + ")");
     if (!h->ready && h->layers[0].n > 0) h->ready = true;  // already set while a serving loop searches
+    return PK_OK;
+}
+
+extern "C" int pk_set_build_row_limit(pk_index* h, int limit) {
+    if (!h) return set_error(PK_ERR_ARG, "null handle");
+    if (limit != 0 && (limit < 2 * h->M || limit > kRowMax - 1))
+        return set_error(PK_ERR_ARG, "row limit must be 0 (511) or in [2M, 511]");
+    h->row_limit = limit;
     return PK_OK;
 }

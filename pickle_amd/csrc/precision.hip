@@ -344,7 +344,7 @@ int pk_index_copy_graph(pk_index* dst, pk_index* src) {
 // their tables, on no more waves than the allocated scratch holds)
 static int rerank_device(pk_index* h, const float* d_queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,
                          int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters,
-                         int vbits_override) {
+                         int vbits_override, bool fast = false) {
     if (!h) return set_error(PK_ERR_ARG, "null handle");
     int rc = check_precision_form(h);
     if (rc) return rc;
@@ -360,8 +360,12 @@ static int rerank_device(pk_index* h, const float* d_queries, int64_t nq, int32_
         return rc;
     // queries -> traversal type (rows of qbytes, as the search reads them)
     if ((rc = quantize_rows(h, d_queries, h->dim, nq, static_cast<uint8_t*>(qc), h->qbytes))) return rc;
-    rc = search_on(h, h->sc, qc, nq, nullptr, nullptr, ef, ef, ef, ef, static_cast<int32_t*>(cids),
-                   static_cast<float*>(cd), static_cast<int32_t*>(cc), d_counters, vbits_override);
+    if (fast)
+        rc = pk_search_fast_device(h, qc, nq, nullptr, nullptr, ef, ef, ef, ef, static_cast<int32_t*>(cids),
+                                   static_cast<float*>(cd), static_cast<int32_t*>(cc), d_counters);
+    else
+        rc = search_on(h, h->sc, qc, nq, nullptr, nullptr, ef, ef, ef, ef, static_cast<int32_t*>(cids),
+                       static_cast<float*>(cd), static_cast<int32_t*>(cc), d_counters, vbits_override);
     if (rc) return rc;
     RerankArgs a{};
     a.queries = d_queries;
@@ -394,6 +398,14 @@ int pk_search_rerank_device(pk_index* h, const float* d_queries, int64_t nq, int
                             int32_t k_stride, int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count,
                             pk_query_counters* d_counters) {
     return rerank_device(h, d_queries, nq, k, ef, k_stride, d_out_ids, d_out_dists, d_out_count, d_counters, 0);
+}
+
+int pk_search_rerank_fast_device(pk_index* h, const float* d_queries, int64_t nq, int32_t k, int32_t ef,
+                                 int32_t k_stride, int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count,
+                                 pk_query_counters* d_counters) {
+    if (ef > 512) return set_error(PK_ERR_ARG, "fast mode takes ef in [k, 512]");
+    return rerank_device(h, d_queries, nq, k, ef, k_stride, d_out_ids, d_out_dists, d_out_count, d_counters, 0,
+                         true);
 }
 
 int pk_search_rerank(pk_index* h, const float* queries, int64_t nq, int32_t k, int32_t ef, int32_t k_stride,

@@ -79,3 +79,29 @@ def test_pmc_summary_calibrates_traffic_and_reads_l2_hits(tmp_path):
     assert rec["fetch_bytes"] == 1000 * 1024 * 2 and rec["write_bytes"] == 300 * 1024
     assert rec["traffic_bytes"] == rec["fetch_bytes"] + rec["write_bytes"]
     assert rec["l2_hit_rate"] == 0.25
+
+
+def test_pmc_summary_keeps_kernels_apart(tmp_path):
+    """A fast-mode line reads only k_search_fast's dispatches, not the exact
+    k_search's of the same process (their short names share a prefix)."""
+    kib = bench.CALIB_BYTES / 1024
+    exact = "void pk::k_search<pk::FloatDist<2, 1, 8>, false>(pk::SearchArgs)"
+    fast = "void pk::k_search_fast<pk::FloatDist<2, 1, 8>, false, 8>(pk::FastArgs)"
+    copy = "pk::k_copy16(void*)"
+    _counter_csv(tmp_path / "f", [(1, exact, "FETCH_SIZE", 5000), (2, fast, "FETCH_SIZE", 700),
+                                  (3, exact, "FETCH_SIZE", 5000), (4, fast, "FETCH_SIZE", 900),
+                                  (5, copy, "FETCH_SIZE", kib / 2)])
+    _counter_csv(tmp_path / "w", [(1, exact, "WRITE_SIZE", 50), (2, fast, "WRITE_SIZE", 1),
+                                  (3, exact, "WRITE_SIZE", 50), (4, fast, "WRITE_SIZE", 3),
+                                  (5, copy, "WRITE_SIZE", kib)])
+    bl = {"steps": 2, "config": {"workload": "w", "ef": 320, "k": 10, "nq": 2, "n": 10, "recall@10": 0.95},
+          "roofline": {"algorithmic_bytes_per_launch": 1024000, "kernel": "k_search_fast"}}
+    (tmp_path / "b.json").write_text(json.dumps(bl) + "\n")
+    out = tmp_path / "p.md"
+    args = type("A", (), dict(bench_json=str(tmp_path / "b.json"), cmd=["cmd"], summarize_trace=None,
+                              summarize_pmc=(str(tmp_path / "f"), str(tmp_path / "w")),
+                              summarize_l2=None, workload="w", out=str(out)))
+    bench.summarize(args)
+    rec = json.loads([x for x in out.read_text().splitlines() if x.startswith("traffic-json: ")][0][14:])
+    assert rec["kernel"] == "k_search_fast"
+    assert rec["fetch_bytes"] == 800 * 1024 * 2 and rec["write_bytes"] == 2 * 1024

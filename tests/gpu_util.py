@@ -17,3 +17,51 @@ def index_from_oracle(g, capacity=None, slack=True):
                        dist=L["dist"][:cap])
     idx.set_enter_level(g.enter_level)
     return idx
+
+
+def oracle_from_index(idx, base, metric, M, efc, levels):
+    """An oracle graph holding the layers the GPU index holds (read back
+    through pk_index_read_layer): rows of D ids (no slack past the last row;
+    use with graphs whose rows stay within D), every node with its vector."""
+    from oracle import oracle as O
+
+    g = O.Graph(base, metric, M, efc, levels)
+    info = idx.info()
+    row_bytes = base.dtype.itemsize * base.shape[1]
+    for l in range(info["nlayers"]):
+        L = idx.read_layer(l)
+        n, D = L["adj"].shape
+        adj = np.full((n + 1, D), -1, np.int32)
+        adj[:n] = L["adj"]
+        dist = np.full((n + 1, D), np.finfo(np.float32).max, np.float32)
+        dist[:n] = L["dist"]
+        data = np.ascontiguousarray(base[L["ext"]]).view(np.uint8).reshape(n, row_bytes)
+        g.set_layer(l, {"cap": n, "D": D, "next_id": n, "adj": adj, "dist": dist,
+                        "deg": L["deg"].astype(np.int16), "heur": L["heur"], "ext": L["ext"],
+                        "data": data, "has_data": np.ones(n, np.uint8)})
+    g.set_enter_level(info["enter_level"])
+    return g
+
+
+def row_structure(idx, level, D_cap=None, overlapped_ok=False):
+    """(rows with an id outside the layer or -1 within the degree, rows with
+    a repeated id, rows longer than D_cap) of one layer.  With overlapped_ok,
+    a repeat does not count in a row whose storage an earlier row reaches
+    (a reference row longer than D continues into the next rows' slots,
+    hnsw_layer.hpp:441-447, and a write there can repeat an id)."""
+    L = idx.read_layer(level)
+    n, D = L["adj"].shape
+    ends = np.arange(n, dtype=np.int64) * D + L["deg"].astype(np.int64)  # flat end of every row
+    bad_id = dup = too_long = 0
+    for v in range(n):
+        d = int(L["deg"][v])
+        if D_cap is not None and d > D_cap:
+            too_long += 1
+        row = L["adj"][v, :min(d, D)]
+        if (row < 0).any() or (row >= n).any():
+            bad_id += 1
+        if len(set(row.tolist())) != len(row):
+            lo = max(0, v - 8)
+            if not (overlapped_ok and (ends[lo:v] > v * D).any()):
+                dup += 1
+    return bad_id, dup, too_long

@@ -1,0 +1,245 @@
+"""Parity at the shapes of the benchmark workloads, at sizes the oracle builds
+in seconds.
+
+* 1M x 128 u8 (L2, M=32, efC=256): search bit-exact with the oracle (ids,
+  distances, counts, work counters) at ef 16..512 on a reference-rows graph
+  with rows grown past 64 slots (the second chunk of a row), and a batch-1
+  build bit-exact with the oracle's build.
+* 1M x 96 f32 unit vectors (IP, M=24, efC=200): the GPU batched build's
+  recall-vs-ef curve within 0.5% of the oracle-built graph's; per-query
+  ef in [16, 512] and k in {1, 10, 100} in one launch, each query bit-exact
+  with the oracle's search at its own k and ef.
+* batched builds keep rows well formed (IsValid, hnsw_layer.hpp:278-295,
+  plus unique ids within a row), also when every insert names one hub.
+* 100M x 128 u8 serving form (M=32): requests of mixed size, k and ef
+  through the serving loop, with streaming inserts, equal the oracle's
+  AnnSearch on the graph the GPU holds.
+The precision form at d=128, M=32 is in test_gpu_precision.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pickle_amd import abi, datasets
+from tests.gpu_util import index_from_oracle, oracle_from_index, row_structure
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def recall(ids, gt, k=10):
+    return float(np.mean([len(set(ids[i, :k]) & set(gt[i, :k])) / k for i in range(len(ids))]))
+
+
+@pytest.fixture(scope="module")
+def u8_graph():
+    n, nq, dim, M, efc = 20000, 200, 128, 32, 256
+    base, queries = datasets.gaussian_mixture(n, nq, dim, 64, 1, kind="u8")
+    levels = O.sample_levels(n, M, seed=7)
+    say("oracle build u8", n, dim, M, efc)
+    g = O.Graph(base, O.L2, M, efc, levels)
+    g.build(np.arange(n, dtype=np.int32))
+    return base, queries, levels, g
+
+
+def test_u8_m32_search_bit_exact(u8_graph):
+    base, queries, levels, g = u8_graph
+    L0 = g.layer(0)
+    assert (L0["deg"] > 64).sum() > 0, "no row grew past one 64-slot chunk"
+    idx = index_from_oracle(g)
+    for k, ef in [(10, 16), (10, 64), (10, 96), (10, 256), (10, 512), (1, 128), (100, 128)]:
+        say("u8 M32 k", k, "ef", ef)
+        want_ids, want_d, want_n, want_c = g.search(queries, k, ef, counters=True)
+        ids, ds, cnt, ctr = idx.search(queries, k, ef, counters=True)
+        np.testing.assert_array_equal(cnt, want_n)
+        np.testing.assert_array_equal(ids, want_ids)
+        np.testing.assert_array_equal(ds, want_d)
+        # per level: [0] layer 0, [1] the upper layers together
+        for f in ("hops", "dists", "neighbors"):
+            want = [want_c[f][0], sum(want_c[f][1:])]
+            assert [int(ctr[f][:, 0].sum()), int(ctr[f][:, 1].sum())] == want, (f, k, ef)
+    idx.close()
+
+
+def test_u8_m32_batch1_build_bit_exact():
+    n, dim, M, efc = 6000, 128, 32, 256
+    base, _ = datasets.gaussian_mixture(n, 1, dim, 64, 11, kind="u8")
+    levels = O.sample_levels(n, M, seed=12)
+    g = O.Graph(base, O.L2, M, efc, levels)
+    g.build(np.arange(n, dtype=np.int32))
+    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
+    idx.build(base, levels=levels, max_batch=1, exact_rows=True)
+    assert idx.info()["enter_level"] == g.enter_level
+    for l in range(g.nlayers):
+        want = g.layer(l)
+        got = idx.read_layer(l)
+        m = want["next_id"]
+        np.testing.assert_array_equal(got["ext"][:m], want["ext"][:m], err_msg=f"layer {l} ext")
+        np.testing.assert_array_equal(got["deg"][:m], want["deg"][:m].astype(np.int32), err_msg=f"layer {l} deg")
+        np.testing.assert_array_equal(got["heur"][:m], want["heur"][:m], err_msg=f"layer {l} heur")
+        D = want["D"]
+        np.testing.assert_array_equal(got["adj"][:m], want["adj"][:m, :D], err_msg=f"layer {l} adj")
+        np.testing.assert_array_equal(got["dist"][:m], want["dist"][:m, :D], err_msg=f"layer {l} dist")
+    idx.close()
+
+
+@pytest.fixture(scope="module")
+def ip_case():
+    n, nq, dim, M, efc = 20000, 500, 96, 24, 200
+    base, queries = datasets.vmf_mixture(n, nq, dim, 256, 2)
+    levels = O.sample_levels(n, M, seed=8)
+    say("oracle build f32 IP", n, dim, M, efc)
+    g = O.Graph(base, O.IP, M, efc, levels)
+    g.build(np.arange(n, dtype=np.int32))
+    gt, _ = O.brute_force(base, queries, 100, O.IP)
+    return base, queries, levels, g, gt
+
+
+def test_f32_ip_m24_batched_build_recall_curve(ip_case):
+    base, queries, levels, g, gt = ip_case
+    n, dim, M, efc = base.shape[0], base.shape[1], 24, 200
+    cpu = index_from_oracle(g)
+    gpu = abi.Index(np.float32, abi.IP, dim, M, efc, n)
+    gpu.build(base, levels=levels, max_batch=4096, exact_rows=True, seed=3)
+    gaps = []
+    for ef in (16, 24, 32, 48, 64, 96, 128, 256, 512):
+        r_cpu = recall(cpu.search(queries, 10, ef)[0], gt)
+        r_gpu = recall(gpu.search(queries, 10, ef)[0], gt)
+        say(f"ef {ef}: oracle-built {r_cpu:.4f} GPU-built {r_gpu:.4f}")
+        gaps.append(r_cpu - r_gpu)
+    assert np.mean(gaps) <= 0.005, gaps
+    # rows of the batched build stay sorted and duplicate-free within their segments
+    og = oracle_from_index(gpu, base, O.IP, M, efc, levels) if gpu.read_layer(0)["deg"].max() <= 2 * M else None
+    if og is not None:
+        for l in range(og.nlayers):
+            assert og.invalid_count(l) == 0, l
+    cpu.close()
+    gpu.close()
+
+
+def test_f32_ip_uneven_k_and_ef_bit_exact(ip_case):
+    base, queries, levels, g, gt = ip_case
+    idx = index_from_oracle(g)
+    rng = np.random.default_rng(4)
+    nq = len(queries)
+    efs = np.exp(rng.uniform(np.log(16), np.log(512), nq)).astype(np.int32)
+    ks = rng.choice([1, 10, 100], nq).astype(np.int32)
+    efs = np.maximum(efs, ks)
+    ids, ds, cnt = idx.search(queries, ks, efs, k_stride=100)
+    for i in range(nq):
+        w_ids, w_d, w_n = g.search(queries[i:i + 1], int(ks[i]), int(efs[i]))
+        assert cnt[i] == w_n[0], i
+        np.testing.assert_array_equal(ids[i, :ks[i]], w_ids[0], err_msg=f"query {i} k {ks[i]} ef {efs[i]}")
+        np.testing.assert_array_equal(ds[i, :ks[i]], w_d[0], err_msg=f"query {i}")
+        assert (ids[i, ks[i]:] == -1).all()
+    # the fast mode on the same launch shape: recall within 0.01 of the exact search
+    f_ids, f_ds, f_cnt = idx.search_fast(queries, ks, efs, k_stride=100)
+    mask = ks >= 10
+    assert recall(f_ids[mask], gt[mask]) >= recall(ids[mask], gt[mask]) - 0.01
+    idx.close()
+
+
+@pytest.mark.parametrize("exact_rows", [False, True])
+def test_batched_build_rows_well_formed(exact_rows):
+    n, dim, M, efc = 30000, 128, 32, 128
+    base, queries = datasets.gaussian_mixture(n, 100, dim, 64, 21, kind="u8")
+    levels = abi.sample_levels(n, M, 21)
+    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
+    idx.set_row_limit(96 if exact_rows else 0)
+    idx.build(base, levels=levels, max_batch=8192, exact_rows=exact_rows, seed=5)
+    info = idx.info()
+    for l in range(info["nlayers"]):
+        D = info["degrees"][l]
+        bad, dup, long_ = row_structure(idx, l, D_cap=(96 if exact_rows else D) if l == 0 else None,
+                                        overlapped_ok=exact_rows)
+        say("layer", l, "rows", info["sizes"][l], "bad ids", bad, "repeats", dup, "too long", long_)
+        assert bad == 0 and dup == 0 and long_ == 0, (l, bad, dup, long_)
+    if not exact_rows:
+        og = oracle_from_index(idx, base, O.L2, M, efc, levels)
+        for l in range(og.nlayers):
+            assert og.invalid_count(l) == 0, l
+    idx.close()
+
+
+def test_hub_contention_build():
+    """Every insert's nearest neighbour is one hub row (the origin, with the
+    other rows on a sphere around it): thousands of AddEdge calls per batch
+    queue on its lock.  The build completes, the hub row stays well formed,
+    and the graph searches as well as the oracle-built one."""
+    n, dim, M, efc = 12000, 16, 16, 100
+    rng = np.random.default_rng(9)
+    x = rng.normal(size=(n, dim))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    base = (x * 100.0).astype(np.float32)
+    base[0] = 0.0  # the hub
+    queries = (rng.normal(size=(200, dim)) * 60.0).astype(np.float32)
+    levels = abi.sample_levels(n, M, 9)
+    levels[0] = levels.max()  # the hub is also the entry point
+    idx = abi.Index(np.float32, abi.L2, dim, M, efc, n)
+    idx.build(base, levels=levels, max_batch=4096, exact_rows=False, seed=9)
+    L0 = idx.read_layer(0)
+    hub = int(np.where(L0["ext"] == 0)[0][0])
+    d = int(L0["deg"][hub])
+    row = L0["adj"][hub, :d]
+    say("hub degree", d, "heuristic", int(L0["heur"][hub]))
+    assert d == 2 * M and len(set(row.tolist())) == d and (row >= 0).all()
+    for l in range(idx.info()["nlayers"]):
+        bad, dup, long_ = row_structure(idx, l)
+        assert bad == 0 and dup == 0, (l, bad, dup)
+    og = oracle_from_index(idx, base, O.L2, M, efc, levels)
+    for l in range(og.nlayers):
+        assert og.invalid_count(l) == 0, l
+    gt, _ = O.brute_force(base, queries, 10, O.L2)
+    ref = O.Graph(base, O.L2, M, efc, levels)
+    ref.build(np.arange(n, dtype=np.int32))
+    r_ref = recall(ref.search(queries, 10, 128)[0], gt)
+    r_gpu = recall(idx.search(queries, 10, 128)[0], gt)
+    say("recall@10 ef 128: oracle-built", r_ref, "GPU-built", r_gpu)
+    assert r_gpu >= r_ref - 0.02
+    idx.close()
+
+
+def test_serving_u8_m32_with_inserts_matches_oracle():
+    n0, extra, dim, M, efc = 15000, 2048, 128, 32, 128
+    base, queries = datasets.gaussian_mixture(n0 + extra, 1500, dim, 1024, 4, kind="u8")
+    levels = abi.sample_levels(n0 + extra, M, 4)
+    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n0 + extra)
+    idx.set_base(base[:n0])
+    idx.init_build(levels)
+    idx.insert(np.arange(n0, dtype=np.int32), max_batch=4096, exact_rows=False)
+    rng = np.random.default_rng(6)
+
+    def requests(qs):
+        out, at = [], 0
+        while at < len(qs):
+            m = int(min(len(qs) - at, np.exp(rng.uniform(0, np.log(300)))))
+            out.append((at, m, int(rng.choice([10, 100])), int(rng.integers(100, 257))))
+            at += m
+        return out
+
+    def check(srv, qs, tag):
+        reqs = requests(qs)
+        tickets = [srv.submit(qs[s:s + m], k, ef) for s, m, k, ef in reqs]
+        got = [srv.wait(t) for t in tickets]
+        og = oracle_from_index(idx, base, O.L2, M, efc, levels)
+        for (s, m, k, ef), (ids, ds, cnt, lat, que) in zip(reqs, got):
+            w_ids, w_d, w_n = og.search(qs[s:s + m], k, ef)
+            np.testing.assert_array_equal(cnt, w_n, err_msg=tag)
+            np.testing.assert_array_equal(ids, w_ids, err_msg=tag)
+            np.testing.assert_array_equal(ds, w_d, err_msg=tag)
+        say(tag, len(reqs), "requests equal the oracle")
+
+    srv = abi.Server(idx, slots=4, slot_queries=1024, max_k=100, max_ef=256)
+    check(srv, queries[:700], "before inserts")
+    for s in range(n0, n0 + extra, 1024):
+        t = srv.insert(np.arange(s, s + 1024, dtype=np.int32), base[s:s + 1024], exact_rows=False)
+        srv.wait(t)
+    check(srv, queries[700:], "after inserts")
+    srv.close()
+    idx.close()

@@ -415,6 +415,23 @@ class Index:
                                                  d_count.ptr if d_count else None,
                                                  d_counters.ptr if d_counters else None))
 
+    def search_rerank_fast(self, queries: np.ndarray, k: int, ef: int, k_stride: int | None = None):
+        """pk_search_rerank_fast_device on host FP32 queries: (ids, dists, counts)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        k_stride = k_stride or k
+        bufs = [DeviceBuffer.from_array(q), DeviceBuffer(nq * k_stride * 4), DeviceBuffer(nq * k_stride * 4),
+                DeviceBuffer(nq * 4)]
+        try:
+            self.search_rerank_fast_device(bufs[0], nq, k, ef, k_stride, bufs[1], bufs[2], bufs[3])
+            self.sync()
+            out = (bufs[1].to_array(np.int32, (nq, k_stride)), bufs[2].to_array(np.float32, (nq, k_stride)),
+                   bufs[3].to_array(np.int32, (nq,)))
+        finally:
+            for b in bufs:
+                b.free()
+        return out
+
     def search_fast(self, queries: np.ndarray, k, ef, k_stride: int | None = None, counters: bool = False):
         """Fast-mode search of host queries: (ids, dists, counts[, counters]).
         k / ef may be ints or per-query arrays."""

@@ -11,9 +11,11 @@ ties by candidate rank.
 int8 traversal distances are exact integers, so ids and distances must match
 bit for bit.  An FP16 copy traverses with its own FP32 summation order (16-byte
 gathers), so its traversal distances agree with the reference order within
-1e-5 relative and an id may differ where two candidates' distances are that
-close: at least 99% of ids must match, and every returned distance must be
-the exact FP32 distance (reference order) of the id returned.
+1e-5 relative: an id may differ from the expected one only where the two
+distances at that rank agree within 1e-5 relative (a near-tie), and every
+returned distance must be the exact FP32 distance (reference order) of the id
+returned.  Both shapes run: d=64, M=12, and the 10M workload's d=128, M=32;
+and both traversals (exact and fast mode) are re-ranked.
 """
 import sys
 
@@ -69,9 +71,25 @@ def expected(gconv, qconv, queries, base, metric, k, ef):
     return ids, ds
 
 
+def near_ties_only(got, want, tag, rtol=1e-5):
+    """Positions where the ids differ hold distances equal within rtol."""
+    gi, gd = got
+    wi, wd = want
+    diff = gi != wi
+    if diff.any():
+        a, b = gd[diff].astype(np.float64), wd[diff].astype(np.float64)
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+        assert rel.max() <= rtol, (tag, int(diff.sum()), float(rel.max()))
+    return int(diff.sum())
+
+
+SHAPES = {"d64": (3000, 64, 64, 12, 60, 10, 40), "d128": (8000, 64, 128, 32, 100, 10, 64)}
+
+
+@pytest.mark.parametrize("shape", ["d64", "d128"])
 @pytest.mark.parametrize("kind,metric", [("f16", O.L2), ("i8", O.L2), ("f16", O.IP), ("i8", O.IP)])
-def test_rerank_matches_oracle(kind, metric):
-    n, nq, dim, M, efc, k, ef = 3000, 64, 64, 12, 60, 10, 40
+def test_rerank_matches_oracle(kind, metric, shape):
+    n, nq, dim, M, efc, k, ef = SHAPES[shape]
     if metric == O.L2:
         base, queries = datasets.gaussian_mixture(n, nq, dim, 16, 31, kind="f32")
     else:
@@ -102,14 +120,20 @@ def test_rerank_matches_oracle(kind, metric):
         np.testing.assert_array_equal(t_ids, o_ids)
         np.testing.assert_array_equal(t_d, o_d)
     else:
-        assert (ids == want_ids).mean() >= 0.99, (ids == want_ids).mean()
+        n_diff = near_ties_only((ids, ds), (want_ids, want_d), "re-ranked")
         exact = np.array([[O.distance(queries[i], base[j], metric) for j in row] for i, row in enumerate(ids)],
                          np.float32)
         np.testing.assert_array_equal(ds, exact)  # the re-rank's FP32 values, reference order
-        assert (t_ids == o_ids).mean() >= 0.99, (t_ids == o_ids).mean()
         ref_t = np.array([[O.distance(q_c[i], base_c[j], metric) for j in row] for i, row in enumerate(t_ids)],
                          np.float32)
         np.testing.assert_allclose(t_d, ref_t, rtol=1e-5, atol=1e-6)
+        near_ties_only((t_ids, ref_t), (o_ids, o_d), "traversal")
+        say("  ids differing at near-ties:", n_diff, "of", ids.size)
+    # the fast traversal, re-ranked the same way: recall-equivalent
+    gt, _ = O.brute_force(base, queries, k, metric)
+    rec = lambda a: np.mean([len(set(x) & set(y)) / k for x, y in zip(a, gt)])
+    f_ids = idx.search_rerank_fast(queries, k, ef)[0]
+    assert rec(f_ids) >= rec(ids) - 0.01, (rec(f_ids), rec(ids))
     idx.close()
     src.close()
 

@@ -160,26 +160,30 @@ struct Beam {
     }
 
     // W <- the 64 * RR smallest keys of W and the sorted batch s (one key
-    // per lane): the smallest 64 of W's last row and s, sorted, then one
-    // bitonic merge of the whole array.  Returns the 64 keys that fall off
-    // the end, sorted: the merged sequence continues with them, so a
-    // duplicate dropped from W is replaced from there.
-    __device__ __forceinline__ uint64_t merge(uint64_t s, int lane) {
+    // per lane), when every key of s is above the keys before the last SUB
+    // rows: those rows and s hold W's tail, so only they merge -- the
+    // smallest 64 of the last row and s, sorted, then one bitonic merge of
+    // the SUB rows.  Returns the 64 keys that fall off the end, sorted: the
+    // merged sequence continues with them, so a duplicate dropped from W is
+    // replaced from there.
+    template <int SUB>
+    __device__ __forceinline__ uint64_t merge_tail(uint64_t s, int lane) {
+        constexpr int R0 = RR - SUB;
         const uint64_t srev = shfl64(s, kWave - 1 - lane);
         if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
         const uint64_t last = key[RR - 1];
         const uint64_t t = min64(last, srev);
         const uint64_t over = bitonic_merge64(max64(last, srev), lane, true);
-        if (RR == 1) {
-            key[0] = bitonic_merge64(t, lane, true);
+        if (SUB == 1) {
+            key[RR - 1] = bitonic_merge64(t, lane, true);
             return over;
         }
         key[RR - 1] = bitonic_merge64(t, lane, false);  // rows ascending, then this row descending
 #pragma unroll
-        for (int J = RR / 2; J >= 1; J >>= 1)
+        for (int J = SUB / 2; J >= 1; J >>= 1)
 #pragma unroll
-            for (int r = 0; r < RR; r++)
-                if ((r & J) == 0) {
+            for (int r = R0; r < RR; r++)
+                if (((r - R0) & J) == 0) {
                     const uint64_t a = key[r], b = key[r + J];
                     key[r] = min64(a, b);
                     key[r + J] = max64(a, b);
@@ -188,53 +192,82 @@ struct Beam {
         // one row at a time: interleaving the rows' exchanges raises register
         // pressure enough to cost occupancy with eight rows
 #pragma unroll
-        for (int r = 0; r < RR; r++) {
+        for (int r = R0; r < RR; r++) {
             key[r] = bitonic_merge64(key[r], lane, true);
             if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
         }
         return over;
     }
 
-    // Over W followed by the overflow row (the sorted continuation from
-    // merge): keeps the first of adjacent copies of one (distance, id) --
-    // the expanded one if any, since it sorts first -- and refills W from
-    // the overflow; returns the copies dropped.  Survivors move left through
-    // LDS (64 * (RR + 1) keys of scratch) -- only when an id the visited
-    // filter forgot came back while still in W.
-    __device__ int dedup(uint64_t over, uint64_t* scratch, int lane) {
-        uint64_t v[RR + 1];
+    // Over the last SUB rows of W followed by the overflow row (the sorted
+    // continuation from merge_tail): keeps the first of adjacent copies of
+    // one (distance, id) -- the expanded one if any, since it sorts first --
+    // and refills W from the overflow; returns the copies dropped.
+    // Survivors move left through LDS (64 * (SUB + 1) keys of scratch) --
+    // only when an id the visited filter forgot came back while in W.
+    template <int SUB>
+    __device__ int dedup_tail(uint64_t over, uint64_t* scratch, int lane) {
+        constexpr int R0 = RR - SUB;
+        uint64_t v[SUB + 1];
 #pragma unroll
-        for (int r = 0; r < RR; r++) v[r] = key[r];
-        v[RR] = over;
-        uint64_t dupm[RR + 1];
+        for (int r = 0; r < SUB; r++) v[r] = key[R0 + r];
+        v[SUB] = over;
+        uint64_t dupm[SUB + 1];
         int total = 0;
 #pragma unroll
-        for (int r = 0; r <= RR; r++) {
+        for (int r = 0; r <= SUB; r++) {
             if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
-            const uint64_t row_before = r > 0 ? readlane64(v[r - 1], kWave - 1) : kKeyInf;
+            const uint64_t row_before =
+                r > 0 ? readlane64(v[r - 1], kWave - 1) : (R0 > 0 ? readlane64(key[R0 > 0 ? R0 - 1 : 0], kWave - 1)
+                                                                  : kKeyInf);
             uint64_t prev = shfl64(v[r], lane > 0 ? lane - 1 : 0);
             if (lane == 0) prev = row_before;
             const bool dup = v[r] != kKeyInf && prev != kKeyInf && (v[r] >> 1) == (prev >> 1);
             dupm[r] = ballot(dup);
-            if (r < RR) total += __builtin_popcountll(dupm[r]);
+            if (r < SUB) total += __builtin_popcountll(dupm[r]);
         }
         // a duplicate inside the overflow alone changes nothing in W
-        if (total == 0 && !(dupm[RR] & 1)) return 0;
-        int before = 0, all = 0;
+        if (total == 0 && !(dupm[SUB] & 1)) return 0;
+        int before = 0;
 #pragma unroll
-        for (int r = 0; r <= RR; r++) {
+        for (int r = 0; r <= SUB; r++) {
             const bool dup = (dupm[r] >> lane) & 1;
             const int np = 64 * r + lane - before - lanes_below(dupm[r]);
             if (!dup) scratch[np] = v[r];
             before += __builtin_popcountll(dupm[r]);
         }
-        all = before;
-        for (int p = 64 * (RR + 1) - all + lane; p < 64 * (RR + 1); p += kWave) scratch[p] = kKeyInf;
+        for (int p = 64 * (SUB + 1) - before + lane; p < 64 * (SUB + 1); p += kWave) scratch[p] = kKeyInf;
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int r = 0; r < RR; r++) key[r] = scratch[64 * r + lane];
+        for (int r = 0; r < SUB; r++) key[R0 + r] = scratch[64 * r + lane];
         __builtin_amdgcn_wave_barrier();
         return total;
+    }
+
+    // merge the sorted batch s into W, touching only the tail rows it can
+    // reach, then drop duplicates
+    __device__ __forceinline__ void merge_dedup(uint64_t s, uint64_t* scratch, int lane) {
+        const uint64_t smin = readlane64(s, 0);
+        // the last key before the last 1, 2, 4 rows
+        const uint64_t b1 = RR >= 2 ? readlane64(key[RR >= 2 ? RR - 2 : 0], kWave - 1) : kKeyInf;
+        const uint64_t b2 = RR >= 4 ? readlane64(key[RR >= 4 ? RR - 3 : 0], kWave - 1) : kKeyInf;
+        const uint64_t b4 = RR >= 8 ? readlane64(key[RR >= 8 ? RR - 5 : 0], kWave - 1) : kKeyInf;
+        if (RR >= 2 && smin > b1) {
+            dedup_tail<1>(merge_tail<1>(s, lane), scratch, lane);
+        } else if constexpr (RR >= 4) {
+            if (smin > b2) {
+                dedup_tail<RR >= 4 ? 2 : 1>(merge_tail<RR >= 4 ? 2 : 1>(s, lane), scratch, lane);
+            } else if constexpr (RR >= 8) {
+                if (smin > b4)
+                    dedup_tail<RR >= 8 ? 4 : 1>(merge_tail<RR >= 8 ? 4 : 1>(s, lane), scratch, lane);
+                else
+                    dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+            } else {
+                dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+            }
+        } else {
+            dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+        }
     }
 
     // Inserts one pending key c at its sorted position, moving the keys
@@ -249,9 +282,11 @@ struct Beam {
             dup |= ballot(key[r] != kKeyInf && (key[r] >> 1) == (c >> 1)) != 0;
         }
         if (dup || pos >= ef) return;
-        uint64_t carry = kKeyInf;  // last key of the row before
+        const int prow = pos >> 6;  // rows before this one do not move
+        uint64_t carry = kKeyInf;   // last key of the row before
 #pragma unroll
         for (int r = 0; r < RR; r++) {
+            if (r < prow) continue;
             const uint64_t last = readlane64(key[r], kWave - 1);
             const uint64_t up = shift_up1(key[r], carry);
             const int p = 64 * r + lane;
@@ -352,7 +387,7 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
         if (pass == 0) continue;
         // (not with eight rows: the second path there costs the kernel a
         // quarter of its occupancy in registers)
-        if (RR < 8 && __builtin_popcountll(pass) <= kInsertMax) {
+        if (__builtin_popcountll(pass) <= kInsertMax) {
             // few accepted: one at a time, each against the beam as it stands
             while (pass) {
                 const int ln = __builtin_ctzll(pass);
@@ -367,9 +402,8 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
         }
         const uint64_t sorted = bitonic_sort64(key, lane);
         if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
-        const uint64_t over = W.merge(sorted, lane);
+        W.merge_dedup(sorted, scratch, lane);
         if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
-        W.dedup(over, scratch, lane);
         W.cap(ef, lane);
         len = W.count();
     }

@@ -637,6 +637,135 @@ def run_uneven(args):
     print(json.dumps(line), flush=True)
 
 
+def library_vbits(ef, n):
+    """The library's visited-table size for an ef (index.hip,
+    default_vbits): by ef, at most twice the capacity."""
+    v = 14 if ef <= 32 else 15 if ef <= 64 else 16 if ef <= 256 else 17 if ef <= 512 else 18
+    cap_bits = 10
+    while cap_bits < 20 and (1 << cap_bits) < 2 * n:
+        cap_bits += 1
+    return min(v, cap_bits)
+
+
+GRID_K = (1, 10, 100)  # benchmark/bench_util.hpp:24
+GRID_EF = (1, 5, 10, 20, 30, 50, 70, 90, 100, 200, 300, 400, 500)  # benchmark/bench_util.hpp:25
+
+
+def run_grid(args):
+    """The reference's own sweep (benchmark/hnsw_pickle.cpp:64-121): every k
+    in {1, 10, 100} against every ef of bench_util.hpp:25, ef < k skipped, on
+    one reference-built graph.  Per (k, ef): recall (%, label matches), the
+    GPU exact and fast searches' QPS over the whole query batch (inputs and
+    outputs in HBM, HIP events), and the reference AnnSearch's QPS and QPS per
+    thread (qps/t, the reference's own figure) on the host cores, with its
+    Profiler hops and distances per query."""
+    from pickle_amd import abi, datasets
+
+    wl = WORKLOADS[args.workload]
+    n, nq = args.n or wl["n"], args.nq or wl["nq"]
+    base, queries, params = datasets.workload(args.workload, n=n, nq=nq)
+    dim, metric, M, efc = base.shape[1], params["metric"], params["M"], params["efc"]
+    dtype_name = {np.dtype(np.uint8): "u8", np.dtype(np.int8): "i8", np.dtype(np.float32): "f32",
+                  np.dtype(np.float16): "f16"}[base.dtype]
+    threads = host_threads()
+    binary, isa = ref_binary()
+    if binary is None:
+        sys.exit("the reference build (oracle/_ref) is missing")
+    work = args.graph_cache or tempfile.mkdtemp(prefix="pk_grid_")
+    os.makedirs(work, exist_ok=True)
+    base_path = os.path.join(work, f"grid_{args.workload}_n{n}.base")
+    graph_path = os.path.join(work, f"grid_{args.workload}_n{n}.graph")
+    base.tofile(base_path)
+    log(f"grid on {args.workload}: n={n} nq={nq}; reference CPU build ({isa}, {threads} threads) ...")
+    binfo = cpu_build(binary, base_path, n, dim, dtype_name, metric, M, efc, threads, graph_path)
+    log(f"built in {binfo['build_seconds']:.1f}s")
+    idx = abi.Index(base.dtype, abi.L2 if metric == "l2" else abi.IP, dim, M, efc, n)
+    idx.set_base(base)
+    idx.load_file(graph_path)
+
+    ns = min(nq, args.gt_queries or 1000)
+    t1 = time.time()
+    gt = host_ground_truth(base, queries[:ns], max(GRID_K), metric)
+    log(f"host ground truth (top {max(GRID_K)}, {ns} queries) in {time.time() - t1:.1f}s")
+    cnq = min(nq, args.cpu_queries or 2000)
+    q_path = os.path.join(work, f"grid_q{cnq}.queries")
+    queries[:cnq].tofile(q_path)
+
+    d_q = abi.DeviceBuffer.from_array(queries)
+    kmax = max(GRID_K)
+    d_ids, d_d, d_c = abi.DeviceBuffer(nq * kmax * 4), abi.DeviceBuffer(nq * kmax * 4), abi.DeviceBuffer(nq * 4)
+
+    def gpu_leg(fast, k, ef):
+        launch = idx.search_fast_device if fast else idx.search_device
+        while True:
+            launch(d_q, nq, k, ef, k, d_ids, d_d, d_c)
+            idx.sync()
+            idx.timer_begin()
+            for _ in range(3):
+                launch(d_q, nq, k, ef, k, d_ids, d_d, d_c)
+            ms = idx.timer_end() / 3
+            flags = idx.device_error()
+            if not fast and flags == 1 and max(vb[0], library_vbits(ef, n)) < 20:
+                # a visited table filled: the next size per wave
+                vb[0] = max(vb[0], library_vbits(ef, n)) + 1
+                idx.set_tuning(0, vb[0], 0)
+                continue
+            break
+        if flags:
+            sys.exit(f"grid: device search failure flags {flags} at k={k} ef={ef}")
+        ids = d_ids.to_array(np.int32, (nq, k))[:ns]
+        return round(100.0 * recall_at_k(ids, gt, k), 2), round(nq / (ms / 1e3), 1)
+
+    rows = []
+    for k in GRID_K:
+        efs = [ef for ef in GRID_EF if ef >= k]  # hnsw_pickle.cpp:74
+        vb = [0]
+        idx.set_tuning(0, 0, 0)
+        out = os.path.join(work, f"grid_cpu_k{k}")
+        st = cpu_search_counters(binary, graph_path, base_path, q_path, n, cnq, dim, dtype_name, k, efs, threads,
+                                 out)
+        for ef, cs in zip(efs, st):
+            cids = read_ref_results(f"{out}.ef{ef}", cnq, k)
+            r_n = min(cnq, ns)
+            c_rec = round(100.0 * recall_at_k(cids[:r_n], gt[:r_n], k), 2)
+            x_rec, x_qps = gpu_leg(False, k, ef)
+            same = float(np.mean(np.all(d_ids.to_array(np.int32, (nq, k))[:cnq] == cids, axis=1)))
+            f_rec, f_qps = gpu_leg(True, k, ef)
+            row = {"k": k, "ef": ef, "gpu_exact": {"recall": x_rec, "qps": x_qps, "ids_equal_reference": round(same, 4)},
+                   "gpu_fast": {"recall": f_rec, "qps": f_qps},
+                   "cpu_reference": {"recall": c_rec, "qps": round(cs["qps"], 1),
+                                     "qps_per_thread": round(cs["qps"] / threads, 1),
+                                     "hop": round(cs["hops"] / cnq, 1), "dist": round(cs["dists"] / cnq, 1),
+                                     "base_hop": round(cs["hops0"] / cnq, 1),
+                                     "base_dist": round(cs["dists0"] / cnq, 1)}}
+            rows.append(row)
+            log(f"k={k:3d} ef={ef:3d} | CPU recall={c_rec:6.2f} qps={cs['qps']:10,.0f} qps/t={cs['qps'] / threads:8,.0f} "
+                f"hop={cs['hops'] / cnq:6.1f} dist={cs['dists'] / cnq:8.1f} | GPU exact recall={x_rec:6.2f} "
+                f"qps={x_qps:12,.0f} ids=ref {same * 100:.1f}% | GPU fast recall={f_rec:6.2f} qps={f_qps:12,.0f}")
+    hostcpu = cpu_info()
+    line = {"metric": "recall and QPS over the reference's k x ef grid (benchmark/hnsw_pickle.cpp:73-121)",
+            "value": None, "unit": "queries/s", "n_gpus": 1, "steps": 3, "warmup": 1, "higher_is_better": True,
+            "dtype": dtype_name, "data": "synthetic",
+            "config": {"workload": wl["desc"], "n": n, "nq": nq, "recall_queries": ns, "cpu_queries": cnq,
+                       "graph": f"reference CPU build ({binfo['build_seconds']:.1f}s, {threads} threads)",
+                       "cpu": {"threads": threads, "isa": isa, "host": hostcpu}},
+            "grid": rows}
+    print(json.dumps(line), flush=True)
+
+
+def cpu_search_counters(binary, graph, base_path, q_path, n, nq, dim, dtype, k, efs, threads, out):
+    """The reference harness's search over a list of ef values: wall time of
+    one timed pass each, then an untimed pass with the Profiler on
+    (benchmark/hnsw_pickle.cpp:79-92)."""
+    cmd = [binary, "search", "--graph", graph, "--base", base_path, "--queries", q_path, "--n", str(n),
+           "--nq", str(nq), "--dim", str(dim), "--dtype", dtype, "--k", str(k), "--ef", ",".join(map(str, efs)),
+           "--threads", str(threads), "--repeat", "1", "--counters", "1", "--out", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference search failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout)
+
+
 def recall_at_k(ids, gt, k):
     """benchmark/hnsw_pickle.cpp:94-112: a returned id counts if it is one of
     the first k ground-truth labels (ties are not credited)."""
@@ -682,6 +811,8 @@ def main():
     ap.add_argument("--M", type=int, default=32, help="with --base-file")
     ap.add_argument("--efc", type=int, default=None, help="efConstruction (with --base-file: 200 by default)")
     ap.add_argument("--serve", action="store_true", help="open-loop serving run through the request loop")
+    ap.add_argument("--grid", action="store_true",
+                    help="the reference's k x ef grid (k 1/10/100, ef 1..500) on a reference-built graph")
     ap.add_argument("--uneven", action="store_true",
                     help="uneven-batch launch (per-query ef / k) on a GPU-built graph, HIP graph replay")
     ap.add_argument("--requests", type=int, default=2000)
@@ -711,6 +842,12 @@ def main():
 
     if args.uneven:
         run_uneven(args)
+        return
+
+    if args.grid:
+        if args.gpus != 1:
+            sys.exit("bench.py measures one GPU only")
+        run_grid(args)
         return
 
     if args.serve:
@@ -870,13 +1007,7 @@ def main():
     vbits = {}
 
     def default_vbits(ef):
-        """The library's visited-table size for an ef (index.hip,
-        default_vbits): by ef, at most twice the capacity."""
-        v = 14 if ef <= 32 else 15 if ef <= 64 else 16 if ef <= 256 else 17 if ef <= 512 else 18
-        cap_bits = 10
-        while cap_bits < 20 and (1 << cap_bits) < 2 * n:
-            cap_bits += 1
-        return min(v, cap_bits)
+        return library_vbits(ef, n)
 
     def sweep(leg, target, efs):
         h = handle(leg)

@@ -19,15 +19,34 @@ def index_from_oracle(g, capacity=None, slack=True):
     return idx
 
 
+def vector_flags(idx):
+    """Per layer, which nodes hold their vector in that layer's copy (the
+    graph file's sidecar, written by pk_index_save_file).  A node that
+    opened a new top level never had AddNode run there (hnsw.hpp:283-297):
+    its copy on those levels is zeros."""
+    import os
+    import tempfile
+
+    from pickle_amd import serializer
+
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "g.bin")
+        idx.save_file(path)
+        gf = serializer.read(path)
+    return [L.has_vector.copy() for L in gf.layers]
+
+
 def oracle_from_index(idx, base, metric, M, efc, levels):
     """An oracle graph holding the layers the GPU index holds (read back
     through pk_index_read_layer): rows of D ids (no slack past the last row;
-    use with graphs whose rows stay within D), every node with its vector."""
+    use with graphs whose rows stay within D), and each layer's vector copy
+    as the index holds it (zeros for a node that opened a new top level)."""
     from oracle import oracle as O
 
     g = O.Graph(base, metric, M, efc, levels)
     info = idx.info()
     row_bytes = base.dtype.itemsize * base.shape[1]
+    has_all = vector_flags(idx)
     for l in range(info["nlayers"]):
         L = idx.read_layer(l)
         n, D = L["adj"].shape
@@ -35,10 +54,12 @@ def oracle_from_index(idx, base, metric, M, efc, levels):
         adj[:n] = L["adj"]
         dist = np.full((n + 1, D), np.finfo(np.float32).max, np.float32)
         dist[:n] = L["dist"]
-        data = np.ascontiguousarray(base[L["ext"]]).view(np.uint8).reshape(n, row_bytes)
+        has = has_all[l][:n].astype(np.uint8)
+        data = np.ascontiguousarray(base[L["ext"]]).view(np.uint8).reshape(n, row_bytes).copy()
+        data[has == 0] = 0
         g.set_layer(l, {"cap": n, "D": D, "next_id": n, "adj": adj, "dist": dist,
                         "deg": L["deg"].astype(np.int16), "heur": L["heur"], "ext": L["ext"],
-                        "data": data, "has_data": np.ones(n, np.uint8)})
+                        "data": data, "has_data": has})
     g.set_enter_level(info["enter_level"])
     return g
 

@@ -804,6 +804,9 @@ def main():
                     help="comma list of low-precision traversal copies (f16, i8) searched + FP32 re-ranked "
                          "beside the FP32 legs (float32 workloads), or none")
     ap.add_argument("--no-autotune", action="store_true", help="time with the default tuning")
+    ap.add_argument("--fast-tuning", default=None, metavar="W,HB,WPG,BPC",
+                    help="fixed fast-mode tuning (width, hash bits, waves per group, workgroups per CU; 0 = "
+                         "automatic) instead of the autotuner's, e.g. the tuning a profiled run chose")
     ap.add_argument("--base-file", default=None, help=".npy / .fbin / .u8bin / .i8bin base vectors instead of "
                                                       "a synthetic workload (with --query-file)")
     ap.add_argument("--query-file", default=None)
@@ -1051,6 +1054,13 @@ def main():
         """The library's autotuner at this ef on up to 16,384 of the queries
         (the exact search's tuning cannot change results; the fast mode's
         width can, and the sweep after it measures the recall it gives)."""
+        if leg != "exact" and args.fast_tuning:
+            w, hb, wpg, bpc = (int(x) for x in args.fast_tuning.split(","))
+            for h in [idx, *tidx.values()]:
+                h.set_fast_tuning(w, hb, wpg, bpc)
+            tuned[leg] = {"width": w, "hash_bits": hb, "waves_per_group": wpg, "blocks_per_cu": bpc, "fixed": True}
+            log(f"[{leg}] fixed tuning {tuned[leg]}")
+            return
         if args.no_autotune:
             return
         ns = min(nq, 16384)
@@ -1136,6 +1146,7 @@ def main():
     ef_m = results[main_leg]["ef"]
     ctr_m = counters(main_leg, ef_m)
     wall, kernel_s = timed(main_leg, ef_m, args.steps)
+    grid, lds = handle(main_leg).last_launch()  # the timed kernel's launch shape
     qps = nq * args.steps / wall
     results[main_leg]["qps"] = round(qps, 1)
     results[main_leg]["kernel_ms"] = round(kernel_s * 1e3, 4)
@@ -1151,7 +1162,6 @@ def main():
     achieved = alg_bytes / kernel_s / 1e9
     khz = abi.wall_clock_khz()
     lat_ms = ctr_m["latency_ticks"].astype(np.float64) / khz
-    grid, lds = idx.last_launch()
     log(f"timed: {args.steps} x {nq} queries at ef={ef_m} ({main_leg}): {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
     # the main leg's results again (the timed launches' outputs)
@@ -1220,7 +1230,9 @@ def main():
            "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),
            "p99_query_latency_ms": round(float(np.percentile(lat_ms, 99)), 4),
            "distance_error_vs_fp64": dist_err,
-           "batch_latency_ms": round(wall / args.steps * 1e3, 4), "legs": results}
+           "batch_latency_ms": round(wall / args.steps * 1e3, 4),
+           "launch": {"kernel": results[main_leg]["kernel"], "grid": grid, "lds_bytes_per_workgroup": lds},
+           "legs": results}
     if tuned:
         cfg["autotuned"] = tuned
     line = {

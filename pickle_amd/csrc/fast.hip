@@ -114,6 +114,16 @@ __device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint64_t first) {
     return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
 }
 
+// lane i gets lane i + 1's key, lane 63 gets `last`: a DPP wave shift
+// left by one (wave_shl:1), the vacated lane 63 keeping the value supplied
+__device__ __forceinline__ uint64_t shift_down1(uint64_t v, uint64_t last) {
+    constexpr int kWaveShl1 = 0x130;
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)last, (int)(uint32_t)v, kWaveShl1, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(last >> 32), (int)(uint32_t)(v >> 32), kWaveShl1, 0xf,
+                                               0xf, false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
 // lane l's key, uniform (v_readlane reads any lane, whatever its exec bit)
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -199,74 +209,74 @@ struct Beam {
         return over;
     }
 
-    // Over the last SUB rows of W followed by the overflow row (the sorted
-    // continuation from merge_tail): keeps the first of adjacent copies of
-    // one (distance, id) -- the expanded one if any, since it sorts first --
-    // and refills W from the overflow; returns the copies dropped.
-    // Survivors move left through LDS (64 * (SUB + 1) keys of scratch) --
-    // only when an id the visited filter forgot came back while in W.
+    // Over the last SUB rows of W: keeps the first of adjacent copies of one
+    // (distance, id) -- the expanded one if any, since it sorts first.  Each
+    // dropped copy moves the keys after it one place toward the front, in
+    // registers (a DPP shift per row), and the overflow row -- the sorted
+    // continuation from merge_tail -- refills W's last position.  Copies
+    // appear only when an id the visited filter forgot came back while in
+    // W, so the loop usually ends at its first scan.  Returns the copies
+    // dropped.
     template <int SUB>
-    __device__ int dedup_tail(uint64_t over, uint64_t* scratch, int lane) {
+    __device__ int dedup_tail(uint64_t over, int lane) {
         constexpr int R0 = RR - SUB;
-        uint64_t v[SUB + 1];
+        const uint64_t before_tail = R0 > 0 ? readlane64(key[R0 > 0 ? R0 - 1 : 0], kWave - 1) : kKeyInf;
+        int dropped = 0;
+        for (;;) {
+            // the first copy among the tail rows: row prow, lane plane
+            int prow = -1, plane = 0;
+            uint64_t carry = before_tail;
 #pragma unroll
-        for (int r = 0; r < SUB; r++) v[r] = key[R0 + r];
-        v[SUB] = over;
-        uint64_t dupm[SUB + 1];
-        int total = 0;
+            for (int r = 0; r < SUB; r++) {
+                if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
+                const uint64_t v = key[R0 + r];
+                const uint64_t prev = shift_up1(v, carry);
+                const bool dup = v != kKeyInf && prev != kKeyInf && (v >> 1) == (prev >> 1);
+                const uint64_t m = ballot(dup);
+                if (m != 0 && prow < 0) {
+                    prow = r;
+                    plane = (int)__builtin_ctzll(m);
+                }
+                carry = readlane64(v, kWave - 1);
+            }
+            if (prow < 0) return dropped;
 #pragma unroll
-        for (int r = 0; r <= SUB; r++) {
-            if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
-            const uint64_t row_before =
-                r > 0 ? readlane64(v[r - 1], kWave - 1) : (R0 > 0 ? readlane64(key[R0 > 0 ? R0 - 1 : 0], kWave - 1)
-                                                                  : kKeyInf);
-            uint64_t prev = shfl64(v[r], lane > 0 ? lane - 1 : 0);
-            if (lane == 0) prev = row_before;
-            const bool dup = v[r] != kKeyInf && prev != kKeyInf && (v[r] >> 1) == (prev >> 1);
-            dupm[r] = ballot(dup);
-            if (r < SUB) total += __builtin_popcountll(dupm[r]);
+            for (int r = 0; r < SUB; r++) {
+                if (r < prow) continue;
+                if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
+                const uint64_t next = r + 1 < SUB ? readlane64(key[R0 + (r + 1 < SUB ? r + 1 : r)], 0)
+                                                  : readlane64(over, 0);
+                const uint64_t sh = shift_down1(key[R0 + r], next);
+                key[R0 + r] = (r == prow && lane < plane) ? key[R0 + r] : sh;
+            }
+            over = shift_down1(over, kKeyInf);
+            dropped++;
         }
-        // a duplicate inside the overflow alone changes nothing in W
-        if (total == 0 && !(dupm[SUB] & 1)) return 0;
-        int before = 0;
-#pragma unroll
-        for (int r = 0; r <= SUB; r++) {
-            const bool dup = (dupm[r] >> lane) & 1;
-            const int np = 64 * r + lane - before - lanes_below(dupm[r]);
-            if (!dup) scratch[np] = v[r];
-            before += __builtin_popcountll(dupm[r]);
-        }
-        for (int p = 64 * (SUB + 1) - before + lane; p < 64 * (SUB + 1); p += kWave) scratch[p] = kKeyInf;
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int r = 0; r < SUB; r++) key[R0 + r] = scratch[64 * r + lane];
-        __builtin_amdgcn_wave_barrier();
-        return total;
     }
 
     // merge the sorted batch s into W, touching only the tail rows it can
     // reach, then drop duplicates
-    __device__ __forceinline__ void merge_dedup(uint64_t s, uint64_t* scratch, int lane) {
+    __device__ __forceinline__ void merge_dedup(uint64_t s, int lane) {
         const uint64_t smin = readlane64(s, 0);
         // the last key before the last 1, 2, 4 rows
         const uint64_t b1 = RR >= 2 ? readlane64(key[RR >= 2 ? RR - 2 : 0], kWave - 1) : kKeyInf;
         const uint64_t b2 = RR >= 4 ? readlane64(key[RR >= 4 ? RR - 3 : 0], kWave - 1) : kKeyInf;
         const uint64_t b4 = RR >= 8 ? readlane64(key[RR >= 8 ? RR - 5 : 0], kWave - 1) : kKeyInf;
         if (RR >= 2 && smin > b1) {
-            dedup_tail<1>(merge_tail<1>(s, lane), scratch, lane);
+            dedup_tail<1>(merge_tail<1>(s, lane), lane);
         } else if constexpr (RR >= 4) {
             if (smin > b2) {
-                dedup_tail<RR >= 4 ? 2 : 1>(merge_tail<RR >= 4 ? 2 : 1>(s, lane), scratch, lane);
+                dedup_tail<RR >= 4 ? 2 : 1>(merge_tail<RR >= 4 ? 2 : 1>(s, lane), lane);
             } else if constexpr (RR >= 8) {
                 if (smin > b4)
-                    dedup_tail<RR >= 8 ? 4 : 1>(merge_tail<RR >= 8 ? 4 : 1>(s, lane), scratch, lane);
+                    dedup_tail<RR >= 8 ? 4 : 1>(merge_tail<RR >= 8 ? 4 : 1>(s, lane), lane);
                 else
-                    dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+                    dedup_tail<RR>(merge_tail<RR>(s, lane), lane);
             } else {
-                dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+                dedup_tail<RR>(merge_tail<RR>(s, lane), lane);
             }
         } else {
-            dedup_tail<RR>(merge_tail<RR>(s, lane), scratch, lane);
+            dedup_tail<RR>(merge_tail<RR>(s, lane), lane);
         }
     }
 
@@ -373,7 +383,7 @@ __device__ int greedy_fast(const DF& df, const LayerDev& L, int vstride, int cur
 // ones merged into the beam, 64 at a time.
 template <class DF, int RR>
 __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, float* cd, int n, Beam<RR>& W, int ef,
-                      int& len, uint64_t* scratch, int lane) {
+                      int& len, int lane) {
     batch_distances(df, L.vec, vstride, cl, cd, n, lane);
     __builtin_amdgcn_wave_barrier();
     for (int c0 = 0; c0 < n; c0 += kWave) {
@@ -385,8 +395,6 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
         }
         uint64_t pass = ballot(key != kKeyInf);
         if (pass == 0) continue;
-        // (not with eight rows: the second path there costs the kernel a
-        // quarter of its occupancy in registers)
         if (__builtin_popcountll(pass) <= kInsertMax) {
             // few accepted: one at a time, each against the beam as it stands
             while (pass) {
@@ -402,7 +410,7 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
         }
         const uint64_t sorted = bitonic_sort64(key, lane);
         if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
-        W.merge_dedup(sorted, scratch, lane);
+        W.merge_dedup(sorted, lane);
         if constexpr (RR >= 8) __builtin_amdgcn_sched_barrier(0);
         W.cap(ef, lane);
         len = W.count();
@@ -411,7 +419,7 @@ __device__ void flush(const DF& df, const LayerDev& L, int vstride, int* cl, flo
 }
 
 template <class DF, bool IS_INT, int RR>
-__global__ __launch_bounds__(256) void k_search_fast(FastArgs A) {
+__global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;
@@ -420,8 +428,7 @@ __global__ __launch_bounds__(256) void k_search_fast(FastArgs A) {
     const int nslot = 1 << A.hash_bits;
     int* cl = reinterpret_cast<int*>(vtab + nslot);
     float* cd = reinterpret_cast<float*>(cl + kFastCap);
-    uint64_t* scratch = reinterpret_cast<uint64_t*>(cd + kFastCap);  // 64 * (RR + 1) keys
-    uint8_t* qst = reinterpret_cast<uint8_t*>(scratch + 64 * (RR + 1));
+    uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kFastCap);
     const bool leader = lane == 0;
     const int g = lane % DF::LPC;
 
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(256) void k_search_fast(FastArgs A) {
                     l0.neighbors += rh.deg;
                     for (int b0 = 0; b0 < rh.deg; b0 += kWave) {
                         if (n + kWave > kFastCap) {
-                            flush(df, L, A.vstride, cl, cd, n, W, ef, len, scratch, lane);
+                            flush(df, L, A.vstride, cl, cd, n, W, ef, len, lane);
                             l0.dists += n;
                             n = 0;
                         }
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(256) void k_search_fast(FastArgs A) {
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (n > 0) {
-                    flush(df, L, A.vstride, cl, cd, n, W, ef, len, scratch, lane);
+                    flush(df, L, A.vstride, cl, cd, n, W, ef, len, lane);
                     l0.dists += n;
                 }
             }
@@ -669,7 +676,7 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     const int width = t.width > 0 ? t.width : 2;
     const int hb = t.hash_bits > 0 ? t.hash_bits : 11;
     const int wpg = t.waves_per_group > 0 ? t.waves_per_group : 2;
-    size_t per_wave = ((size_t)4 << hb) + (size_t)kFastCap * 8 + (size_t)64 * (rr + 1) * 8 + h->vstride;
+    size_t per_wave = ((size_t)4 << hb) + (size_t)kFastCap * 8 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
     const size_t lds = per_wave * wpg;
     if (lds > 160 * 1024) return set_error(PK_ERR_CAPACITY, "fast search needs too much LDS per workgroup");

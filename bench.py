@@ -346,7 +346,8 @@ def run_serving(args):
     shares = [float(x) for x in str(args.insert_share).split(",")]
     n_ins_all = [int(round(n_req * sh)) if sh > 0 else 0 for sh in shares]
     n_ins_all = [max(1, x) if sh > 0 else 0 for x, sh in zip(n_ins_all, shares)]
-    extra = sum(n_ins_all) * 1024
+    n_ranges = 2 if args.target_recall else 1  # the mixes run once per ef range
+    extra = n_ranges * sum(n_ins_all) * 1024
     streamed = n + extra >= STREAM_ROWS
     if streamed:
         # rows are generated on a second thread while earlier ones are uploaded
@@ -359,6 +360,8 @@ def run_serving(args):
         dim, dtype = base.shape[1], base.dtype
     mcode = abi.L2 if params["metric"] == "l2" else abi.IP
     idx = abi.Index(dtype, mcode, dim, params["M"], params["efc"], n + extra)
+    if args.row_limit:
+        idx.set_row_limit(args.row_limit)
     levels = abi.sample_levels(n + extra, params["M"], args.seed)
     tb = time.perf_counter()
     if streamed:
@@ -405,8 +408,10 @@ def run_serving(args):
                 log(f"  inserted {min(n, s0 + chunk):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
         extra_rows = base[n:]
     build_s = time.perf_counter() - tb
-    log(f"serving index: {n} x {dim} {dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved"
-        f"{', rows capped at D' if args.capped_rows else ''})")
+    rows_desc = ("rows capped at D" if args.capped_rows else
+                 f"reference rows up to {args.row_limit or 511} ids")
+    log(f"serving index: {n} x {dim} {dtype} built on the GPU in {build_s:.1f}s (+{extra} rows reserved, "
+        f"{rows_desc})")
 
     # recall of the served index: exact ground truth on the matrix cores
     # (int8 MFMA for u8 rows) for 1,000 queries, recall@10 by ef
@@ -415,35 +420,62 @@ def run_serving(args):
     gt_ids = idx.brute_force(gq)[0][:, :10]
     gt_s = time.perf_counter() - tg0
     recall_by_ef = {}
-    for ef in (32, 64, 128, 256):
+    for ef in (32, 64, 128, 256, 384, 512, 630):
         ids = idx.search(gq, 10, ef)[0]
         recall_by_ef[ef] = round(recall_at_k(ids, gt_ids, 10), 4)
     log(f"recall@10 by ef {recall_by_ef} (ground truth for {len(gq)} queries in {gt_s:.2f}s)")
 
-    # capacity probe: one slot-sized launch at the mean ef
+    # ef ranges of the mixes: the workload's [16, 256], and with
+    # --target-recall one fixed ef, the smallest whose recall@10 reaches it
+    ranges = [(16, 256)]
+    ef_target = None
+    if args.target_recall:
+        lo, hi = 1, 630
+        for ef in sorted(recall_by_ef):
+            if recall_by_ef[ef] >= args.target_recall:
+                hi = ef
+                break
+            lo = ef
+        while hi - lo > max(4, hi // 32):  # bisect between the measured points
+            mid = (lo + hi) // 2
+            r = recall_at_k(idx.search(gq, 10, mid)[0], gt_ids, 10)
+            recall_by_ef[mid] = round(r, 4)
+            lo, hi = (mid, hi) if r < args.target_recall else (lo, mid)
+        ef_target = hi
+        ranges.append((ef_target, ef_target))
+        log(f"ef {ef_target}: recall@10 {recall_by_ef.get(ef_target)} >= {args.target_recall}")
+    max_ef = max(r[1] for r in ranges)
+
+    # capacity probe per range: one slot-sized launch at the range's mean ef
     d_q = abi.DeviceBuffer.from_array(queries[:4096])
     d_i, d_d, d_c = abi.DeviceBuffer(4096 * 100 * 4), abi.DeviceBuffer(4096 * 100 * 4), abi.DeviceBuffer(4096 * 4)
-    for _ in range(2):
-        idx.timer_begin()
-        idx.search_device(d_q, 4096, 10, 136, 10, d_i, d_d, d_c, None)
-        cap_ms = idx.timer_end()
-    cap_qps = 4096 / (cap_ms / 1e3)
     mean_size = 4095 / np.log(4096)
-    rate = args.load * cap_qps / mean_size
-    log(f"capacity ~{cap_qps:,.0f} QPS at ef=136; offered load {args.load:.2f} -> {rate:,.1f} requests/s")
+    rates = {}
+    for lo, hi in ranges:
+        ef_mean = (lo + hi) // 2
+        for _ in range(2):
+            idx.timer_begin()
+            idx.search_device(d_q, 4096, 10, ef_mean, 10, d_i, d_d, d_c, None)
+            cap_ms = idx.timer_end()
+        cap_qps = 4096 / (cap_ms / 1e3)
+        rates[(lo, hi)] = (args.load * cap_qps / mean_size, cap_qps, ef_mean)
+        log(f"capacity ~{cap_qps:,.0f} QPS at ef={ef_mean}; offered load {args.load:.2f} -> "
+            f"{rates[(lo, hi)][0]:,.1f} requests/s")
 
     rng = np.random.default_rng(args.seed)
     ins_at = n
     mixes = []
     slog = []
-    for mix, (share, n_ins) in enumerate(zip(shares, n_ins_all)):
+    plan = [(r, sh, ni) for r in ranges for sh, ni in zip(shares, n_ins_all)]
+    for mix, ((ef_lo, ef_hi), share, n_ins) in enumerate(plan):
+        rate, cap_qps, ef_mean = rates[(ef_lo, ef_hi)]
         kinds = rng.permutation(np.r_[np.ones(n_ins, bool), np.zeros(n_req - n_ins, bool)])
         gaps = rng.exponential(1.0 / rate, n_req)
         sizes = np.minimum(4096, np.floor(np.exp(rng.uniform(0, np.log(4096), n_req)))).astype(int)
-        efs = rng.integers(16, 257, n_req)
+        efs = rng.integers(ef_lo, ef_hi + 1, n_req)
         ks = rng.choice([10, 100], n_req)
         efs = np.maximum(efs, ks)
-        srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=256)
+        srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=max(max_ef, 100))
         tickets = []
         done = []
         t0 = time.perf_counter()
@@ -491,13 +523,16 @@ def run_serving(args):
         lat_s = np.array([d[2] for d in done if d[0] == "search"]) / 1e3
         lat_i = np.array([d[2] for d in done if d[0] == "insert"]) / 1e3
         nq_done = sum(d[1] for d in done if d[0] == "search")
-        res = {"insert_share": share, "requests": n_req, "inserts": n_ins, "qps": round(nq_done / wall, 1),
+        res = {"ef": f"uniform [{ef_lo}, {ef_hi}]" if ef_lo < ef_hi else ef_lo,
+               "recall@10_at_ef": recall_by_ef.get(ef_lo) if ef_lo == ef_hi else None,
+               "offered_requests_per_s": round(rate, 1), "capacity_qps_probe": round(cap_qps, 1),
+               "insert_share": share, "requests": n_req, "inserts": n_ins, "qps": round(nq_done / wall, 1),
                "p50_request_latency_ms": round(float(np.percentile(lat_s, 50)), 3),
                "p99_request_latency_ms": round(float(np.percentile(lat_s, 99)), 3),
                "p50_insert_latency_ms": round(float(np.percentile(lat_i, 50)), 3) if len(lat_i) else None,
                "queries": int(nq_done), "wall_s": round(wall, 3), "rows_after": int(ins_at), "loop": st}
         mixes.append(res)
-        log(f"mix {mix} (insert share {share}): served {nq_done} queries in {n_req - n_ins} requests + {n_ins} "
+        log(f"mix {mix} (ef {res['ef']}, insert share {share}): served {nq_done} queries in {n_req - n_ins} requests + {n_ins} "
             f"inserts in {wall:.2f}s: {res['qps']:,.0f} QPS, request latency p50 {res['p50_request_latency_ms']} ms, "
             f"p99 {res['p99_request_latency_ms']} ms; loop {st}")
     if args.serve_log:
@@ -512,11 +547,12 @@ def run_serving(args):
         "vs_baseline": None, "dtype": str(dtype), "data": "synthetic",
         "config": {"workload": f"{wl['desc']} (n={n}, GPU-built, +{extra} streamed)", "requests": n_req,
                    "insert_batch": 1024, "slots": args.slots, "offered_load": args.load,
-                   "arrival_rate_per_s": round(rate, 1), "request_sizes": "log-uniform [1, 4096]",
-                   "ef": "uniform [16, 256]", "k": "{10, 100}",
+                   "arrival_rate_per_s": first["offered_requests_per_s"], "request_sizes": "log-uniform [1, 4096]",
+                   "ef": "uniform [16, 256]" + (f"; then fixed at {ef_target}, the smallest ef with recall@10 >= "
+                                                f"{args.target_recall}" if ef_target else ""), "k": "{10, 100}",
                    "p50_request_latency_ms": first["p50_request_latency_ms"],
                    "p99_request_latency_ms": first["p99_request_latency_ms"],
-                   "mixes": mixes, "build_seconds": round(build_s, 2), "rows_capped_at_D": bool(args.capped_rows),
+                   "mixes": mixes, "build_seconds": round(build_s, 2), "rows": rows_desc,
                    "recall@10_by_ef": recall_by_ef, "ground_truth_seconds": round(gt_s, 3)},
         "roofline": None, "cpu_baseline": None,
     }
@@ -826,6 +862,8 @@ def main():
     ap.add_argument("--insert-share", default="0,0.05",
                     help="comma list of insert shares, one serving run each on the same index")
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
+    ap.add_argument("--target-recall", type=float, default=None,
+                    help="serving: also run the mixes at the smallest ef whose recall@10 reaches this")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--serve-log", default=None, help="per-request latencies (JSON lines) of a serving run")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR",

@@ -172,8 +172,8 @@ int pk_search_fast_device(pk_index *h, const void *d_queries, int64_t nq, const 
                           int32_t *d_out_count, pk_query_counters *d_counters);
 
 /* Fast-mode tuning (0 = automatic for each): width 1..4 entries expanded
- * per round (default 2), hash_bits 8..14 (log2 slots of each query's LDS
- * visited filter, default 11), waves_per_group 1, 2 or 4 (queries per
+ * per round (default 2), hash_bits 8..14 (each query's LDS visited filter
+ * takes 4 << hash_bits bytes: 2^(hash_bits + 1) 16-bit entries; default 11), waves_per_group 1, 2 or 4 (queries per
  * workgroup, default 2), blocks_per_cu (cap on resident workgroups).  No
  * tuning changes a result except width. */
 int pk_set_fast_tuning(pk_index *h, int width, int hash_bits, int waves_per_group, int blocks_per_cu);

@@ -18,8 +18,13 @@
  *     the reference, include/util.hpp:34-64);
  *   - Save / Load use the reference serializer layout
  *     (include/serializer.hpp:27-81) plus a "<path>.v2meta" sidecar.
- * Calls on one graph are serialised by a mutex, so AnnSearch(span) may be
- * called from many host threads as in the reference.
+ * AnnSearch(span) may be called from many host threads at once, as the
+ * reference's callers do inside OpenMP loops: such calls share the graph and
+ * go through a serving loop on the index (pk_server_*), which packs the
+ * queries waiting when a launch slot frees into one launch, so concurrent
+ * single-query calls run as batches.  Their results equal a batch search's.
+ * Calls that change the graph (InitBuffer, Build, Insert, InsertBatch, Load)
+ * wait for the searches in flight and retire that loop first.
  */
 #ifndef PICKLE_GPU_HPP
 #define PICKLE_GPU_HPP
@@ -30,6 +35,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <random>
 #include <span>
 #include <stdexcept>
@@ -151,7 +157,8 @@ class HNSWGraph {
     /* include/hnsw.hpp:222-245: fixes the capacity; the layers are sized
      * when the element type and width are known (first Insert). */
     void InitBuffer(size_t max_nodes) {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        retire_server();
         m_capacity = max_nodes;
         m_h.reset();
     }
@@ -159,7 +166,8 @@ class HNSWGraph {
     /* include/hnsw.hpp:497-507: rows of `data` are indexed by external id */
     template <class T, size_t Dim = std::dynamic_extent>
     void Build(const std::vector<external_id_t>& ext_ids, const NDArray& data) {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        retire_server();
         check_type<T>(data.m_dtype);
         if (Dim != std::dynamic_extent && data.m_shape[1] != Dim) throw Error(PK_ERR_ARG, "row width != Dim");
         m_capacity = ext_ids.size();
@@ -173,7 +181,8 @@ class HNSWGraph {
     /* include/hnsw.hpp:247-302 (one node; InsertBatch is the fast path) */
     template <class T, size_t Dim>
     void Insert(external_id_t ext_id, std::span<const T, Dim> query) {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        retire_server();
         if (!m_h) {
             create(dtype_of<T>::value, query.size());
             init_levels();
@@ -187,7 +196,8 @@ class HNSWGraph {
     /* many Insert calls at once: rows[i] is the vector of ext_ids[i] */
     template <class T>
     void InsertBatch(const std::vector<external_id_t>& ext_ids, const NDArray& rows) {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        retire_server();
         if (!m_h) {
             create(dtype_of<T>::value, rows.m_shape[1]);
             init_levels();
@@ -199,21 +209,42 @@ class HNSWGraph {
         check(pk_index_insert(m_h.get(), ext_ids.data(), (int64_t)ext_ids.size(), m_batch, m_flags));
     }
 
-    /* include/hnsw.hpp:509-525 */
+    /* include/hnsw.hpp:509-525.  Without metrics (and with k <= ef <= 630)
+     * the query goes through the graph's serving loop, so calls from many
+     * threads at once run as batched launches; with metrics it runs alone on
+     * the handle's stream, whose counters LastCounters() reports. */
     template <class T, size_t Dim, bool collect_metric = false>
     std::vector<Entry> AnnSearch(int top_k, int search_ef, std::span<const T, Dim> query) const {
-        std::lock_guard<std::mutex> g(m_mu);
-        ready();
-        check_type<T>(m_dtype);
-        if (query.size() != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
-        std::vector<std::vector<Entry>> r = search(top_k, search_ef, query.data(), 1, collect_metric);
-        return std::move(r[0]);
+        if (top_k < 1 || search_ef < 1) throw Error(PK_ERR_ARG, "k and ef must be positive");
+        if (collect_metric || search_ef < top_k || search_ef > kServeMaxEf) {
+            std::shared_lock<std::shared_mutex> g(m_mu);
+            std::lock_guard<std::mutex> d(m_direct);
+            ready();
+            check_type<T>(m_dtype);
+            if (query.size() != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
+            std::vector<std::vector<Entry>> r = search(top_k, search_ef, query.data(), 1, collect_metric);
+            return std::move(r[0]);
+        }
+        for (;;) {
+            {
+                std::shared_lock<std::shared_mutex> g(m_mu);
+                ready();
+                check_type<T>(m_dtype);
+                if (query.size() != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
+                if (m_server && top_k <= m_serve_k) return serve_one(top_k, search_ef, query.data());
+            }
+            // no loop yet, or one whose result rows are narrower than k
+            std::unique_lock<std::shared_mutex> x(m_mu);
+            ready();
+            if (!m_server || top_k > m_serve_k) start_server(top_k);
+        }
     }
 
     /* include/hnsw.hpp:528-541 */
     template <class T, size_t Dim, bool collect_metric = false>
     std::vector<std::vector<Entry>> AnnSearch(int top_k, int search_ef, NDArray query) const {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::shared_lock<std::shared_mutex> g(m_mu);
+        std::lock_guard<std::mutex> d(m_direct);
         ready();
         check_type<T>(query.m_dtype);
         if (query.m_shape[1] != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
@@ -223,14 +254,16 @@ class HNSWGraph {
 
     /* serializer layout (include/serializer.hpp:27-81) + sidecar */
     void Save(const std::string& path) const {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::shared_lock<std::shared_mutex> g(m_mu);
+        std::lock_guard<std::mutex> d(m_direct);
         ready();
         check(pk_index_save_file(m_h.get(), path.c_str()));
     }
     /* the file holds the graph; `data` holds the vectors by external id */
     template <class T>
     void Load(const std::string& path, const NDArray& data) {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        retire_server();
         check_type<T>(data.m_dtype);
         m_capacity = data.m_shape[0];
         create(dtype_of<T>::value, data.m_shape[1]);
@@ -239,7 +272,7 @@ class HNSWGraph {
     }
 
     int GetEnterLevel() const {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::shared_lock<std::shared_mutex> g(m_mu);
         ready();
         int nl = 0, el = 0;
         check(pk_index_info(m_h.get(), &nl, &el, nullptr, nullptr));
@@ -247,7 +280,7 @@ class HNSWGraph {
     }
     /* nodes per layer */
     std::vector<int64_t> LayerSizes() const {
-        std::lock_guard<std::mutex> g(m_mu);
+        std::shared_lock<std::shared_mutex> g(m_mu);
         ready();
         int nl = 0, el = 0;
         std::vector<int64_t> sizes(PK_MAX_LEVELS, 0);
@@ -292,6 +325,42 @@ class HNSWGraph {
         check(pk_sample_levels((int64_t)m_capacity, (int)m_max_degree, seed, levels.data()));
         check(pk_index_init_build(m_h.get(), levels.data()));
     }
+    // the serving loop behind concurrent AnnSearch(span) calls: result rows
+    // of `k` entries (at least 100), any ef in [k, 630]
+    void start_server(int k) const {
+        retire_server();
+        const int kk = std::min(kServeMaxEf, std::max({k, 100, m_serve_k}));
+        pk_server* s = nullptr;
+        check(pk_server_create(m_h.get(), kServeSlots, kServeQueries, kk, kServeMaxEf, &s));
+        m_server.reset(s);
+        m_serve_k = kk;
+    }
+    void retire_server() const {
+        if (m_server) {
+            pk_server* s = m_server.release();
+            check(pk_server_destroy(s));
+        }
+    }
+    std::vector<Entry> serve_one(int k, int ef, const void* q) const {
+        std::vector<int32_t> ids(k);
+        std::vector<float> ds(k);
+        int32_t cnt = 0;
+        int64_t ticket = 0;
+        check(pk_server_submit(m_server.get(), q, 1, k, ef, ids.data(), ds.data(), &cnt, &ticket));
+        check(pk_server_wait(m_server.get(), ticket, -1, nullptr, nullptr));
+        if (cnt < 0) throw Error(PK_ERR_DEVICE, "query failed on the device");
+        std::vector<Entry> out;
+        out.reserve(cnt);
+        for (int j = 0; j < cnt; j++) out.emplace_back(ds[j], ids[j]);
+        return out;
+    }
+    struct ServerFree {
+        void operator()(pk_server* s) const { (void)pk_server_destroy(s); }
+    };
+    static constexpr int kServeMaxEf = 630;  // the largest ef a search supports
+    static constexpr int kServeSlots = 4;
+    static constexpr int kServeQueries = 1024;
+
     std::vector<std::vector<Entry>> search(int k, int ef, const void* q, size_t nq, bool collect) const {
         if (k < 1 || ef < 1) throw Error(PK_ERR_ARG, "k and ef must be positive");
         std::vector<int32_t> ids(nq * k), cnt(nq);
@@ -316,7 +385,13 @@ class HNSWGraph {
     uint64_t m_seed{0};
     std::shared_ptr<pk_index> m_h;
     mutable std::vector<pk_query_counters> m_counters;
-    mutable std::mutex m_mu;
+    // shared by searches; exclusive for calls that change the graph
+    mutable std::shared_mutex m_mu;
+    // calls that use the handle's own stream and scratch run one at a time
+    mutable std::mutex m_direct;
+    // declared after m_h: retired before the index it searches
+    mutable std::unique_ptr<pk_server, ServerFree> m_server;
+    mutable int m_serve_k{0};
 };
 
 }  // namespace pickle_gpu

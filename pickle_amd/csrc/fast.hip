@@ -22,9 +22,13 @@
 //     bitonic network and merged into W with a bitonic merge (the 64
 //     smallest of W's last row and the sorted batch, then a merge of the
 //     whole array), keeping the ef smallest.
-//   * The visited filter is a hash table in LDS (2^hash_bits ids per wave)
-//     whose slots are overwritten on collision, so it may forget an id.  A
-//     forgotten id is only evaluated again: if it is still in W, the merge
+//   * The visited filter is a hash table in LDS of 16-bit entries: a
+//     bijective hash of the id (an odd multiplier modulo 2^b, b the bits of
+//     the index capacity) names a slot by its high bits and is stored as a
+//     tag of its low bits, so a slot and its tag identify one id exactly.
+//     A slot is overwritten on collision, so the filter may forget an id,
+//     but never takes an id it has not seen for one it has.  A forgotten id
+//     is only evaluated again: if it is still in W, the merge
 //     places both copies side by side and one is dropped (keeping the
 //     expanded mark); if it left W, its distance is at or above W's
 //     farthest and the accept rule rejects it.  Results therefore do not
@@ -67,7 +71,9 @@ struct FastArgs {
     int32_t* out_count;
     pk_query_counters* ctr;
     int32_t width;      // entries expanded per round (1..4)
-    int32_t hash_bits;  // log2 slots of the LDS visited filter per wave
+    int32_t slot_bits;  // log2 entries of the LDS visited filter per wave
+    int32_t tag_bits;   // id bits kept in an entry (<= 15), b - slot_bits
+    uint32_t id_mask;   // 2^b - 1
     int32_t lds_per_wave;
     int32_t max_rounds;  // per query; past it the query fails (flag 1)
     int32_t* qcounter;
@@ -329,11 +335,19 @@ struct Beam {
     }
 };
 
-// LDS visited filter: test-and-set by exchange; a collision overwrites
-__device__ __forceinline__ bool filter_new(uint32_t* tab, int bits, int id) {
-    const uint32_t slot = ((uint32_t)id * 0x9E3779B1u) >> (32 - bits);
-    const uint32_t old = atomicExch(&tab[slot], (uint32_t)id + 1u);
-    return old != (uint32_t)id + 1u;
+// LDS visited filter: h = id * odd mod 2^b is a bijection of the ids below
+// 2^b; its high bits pick the slot and its low bits, with the top bit set
+// (an empty slot holds 0), are the entry.  Test and set: a lane reads its
+// slot, then writes its entry.  Two lanes of one instruction with the same
+// id both report it new (it is evaluated twice and the beam's merge drops
+// the copy); a collision overwrites the older entry.
+__device__ __forceinline__ bool filter_new(uint16_t* tab, uint32_t id_mask, int tag_bits, int id) {
+    const uint32_t h = ((uint32_t)id * 0x9E3779B1u) & id_mask;
+    const uint32_t slot = h >> tag_bits;
+    const uint16_t tag = (uint16_t)((h & ((1u << tag_bits) - 1u)) | 0x8000u);
+    const uint16_t old = tab[slot];
+    tab[slot] = tag;
+    return old != tag;
 }
 
 // SlideLayer without a visited set (a node seen before cannot be strictly
@@ -424,9 +438,9 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
     const int lane = lane_id();
     const int w = threadIdx.x >> 6;
     uint8_t* my = smem + (size_t)w * A.lds_per_wave;
-    uint32_t* vtab = reinterpret_cast<uint32_t*>(my);
-    const int nslot = 1 << A.hash_bits;
-    int* cl = reinterpret_cast<int*>(vtab + nslot);
+    uint16_t* vtab = reinterpret_cast<uint16_t*>(my);
+    const int nslot = 1 << A.slot_bits;
+    int* cl = reinterpret_cast<int*>(vtab + (nslot < 8 ? 8 : nslot));
     float* cd = reinterpret_cast<float*>(cl + kFastCap);
     uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kFastCap);
     const bool leader = lane == 0;
@@ -460,7 +474,7 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
         if (!fail) {
             // an empty filter for this query
             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-            for (int i = lane; i < nslot / 4; i += kWave)
+            for (int i = lane; i < (nslot + 7) / 8; i += kWave)
                 reinterpret_cast<u32x4*>(vtab)[i] = u32x4{0u, 0u, 0u, 0u};
             __builtin_amdgcn_wave_barrier();
             const LayerDev L = A.layers[0];
@@ -468,7 +482,7 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
             l0.dists++;
             if (leader) {
                 W.key[0] = make_key(d0, cur);
-                filter_new(vtab, A.hash_bits, cur);
+                filter_new(vtab, A.id_mask, A.tag_bits, cur);
             }
             len = 1;
             __builtin_amdgcn_wave_barrier();
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
                         const bool inrow = j < rh.deg;
                         const int id = b0 == 0 ? rh.first : (inrow ? gbl(L.adj)[rh.base + j] : -1);
                         const bool valid = inrow && id >= 0 && (int64_t)id < L.n;
-                        const bool isnew = valid && filter_new(vtab, A.hash_bits, id);
+                        const bool isnew = valid && filter_new(vtab, A.id_mask, A.tag_bits, id);
                         const uint64_t nm = ballot(isnew);
                         if (isnew) cl[n + lanes_below(nm)] = id;
                         n += __builtin_popcountll(nm);
@@ -676,7 +690,14 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     const int width = t.width > 0 ? t.width : 2;
     const int hb = t.hash_bits > 0 ? t.hash_bits : 11;
     const int wpg = t.waves_per_group > 0 ? t.waves_per_group : 2;
-    size_t per_wave = ((size_t)4 << hb) + (size_t)kFastCap * 8 + h->vstride;
+    // the filter takes 4 << hb bytes: 2^(hb + 1) entries of 16 bits, each
+    // keeping b - (hb + 1) <= 15 bits of the hashed id (more entries when the
+    // capacity needs it); with no more ids than entries every id has its own
+    int b = 1;
+    while (b < 27 && ((int64_t)1 << b) < h->capacity) b++;
+    int sb = std::max(hb + 1, b - 15);
+    if (b < sb) b = sb;
+    size_t per_wave = ((size_t)2 << std::max(sb, 3)) + (size_t)kFastCap * 8 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
     const size_t lds = per_wave * wpg;
     if (lds > 160 * 1024) return set_error(PK_ERR_CAPACITY, "fast search needs too much LDS per workgroup");
@@ -704,7 +725,9 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     a.out_count = d_out_count;
     a.ctr = d_counters;
     a.width = width;
-    a.hash_bits = hb;
+    a.slot_bits = sb;
+    a.tag_bits = b - sb;
+    a.id_mask = b >= 32 ? 0xffffffffu : ((1u << b) - 1u);
     a.lds_per_wave = (int)per_wave;
     a.max_rounds = (int)std::min<int64_t>(1 << 30, 4 * h->capacity + 64);
     a.qcounter = h->sc.qcounter;

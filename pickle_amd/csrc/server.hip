@@ -145,9 +145,11 @@ int alloc_slot(pk_server* S, Slot& s) {
 int launch_slot(pk_server* S, Slot& s) {
     const size_t rb = (size_t)S->h->qbytes;
     int64_t nq = 0;
+    int ef_launch = 1;  // the launch's LDS and visited tables follow its largest ef, not the server's limit
     s.first.clear();
     for (Request* r : s.reqs) {
         s.first.push_back(nq);
+        ef_launch = std::max(ef_launch, (int)r->ef);
         std::memcpy(s.h_q + nq * rb, r->queries.data(), r->nq * rb);
         for (int64_t i = 0; i < r->nq; i++) {
             s.h_k[nq + i] = r->k;
@@ -159,7 +161,7 @@ int launch_slot(pk_server* S, Slot& s) {
     SCHK(hipMemcpyAsync(s.d_q, s.h_q, nq * rb, hipMemcpyHostToDevice, st));
     SCHK(hipMemcpyAsync(s.d_k, s.h_k, nq * 4, hipMemcpyHostToDevice, st));
     SCHK(hipMemcpyAsync(s.d_ef, s.h_ef, nq * 4, hipMemcpyHostToDevice, st));
-    int rc = search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, S->max_ef, S->max_k, s.d_ids, s.d_d, s.d_cnt,
+    int rc = search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, ef_launch, S->max_k, s.d_ids, s.d_d, s.d_cnt,
                        nullptr, s.vbits);
     if (rc) return rc;
     SCHK(hipMemcpyAsync(s.h_ids, s.d_ids, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));

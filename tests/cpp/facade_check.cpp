@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <thread>
 
 #include "pickle_gpu.hpp"
 
@@ -63,6 +64,42 @@ static int compare_single(const HNSWGraph& g, const NDArray& q, size_t nq, int k
     return 0;
 }
 
+// single-query AnnSearch from 8 host threads at once (a caller's OpenMP loop
+// over queries): every result equals the batch's
+template <class T>
+static int compare_threads(const HNSWGraph& g, const NDArray& q, size_t nq, int k, int ef,
+                           const std::vector<std::vector<Entry>>& batch) {
+    const size_t m = std::min<size_t>(nq, 512);
+    const int nt = 8;
+    std::vector<int> bad(nt, -1);
+    std::vector<std::string> err(nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (size_t i = t; i < m; i += nt)
+                    if (g.AnnSearch<T, std::dynamic_extent>(k, ef, q.get_span<T>(i)) != batch[i]) {
+                        bad[t] = (int)i;
+                        return;
+                    }
+            } catch (const std::exception& e) {
+                err[t] = e.what();
+            }
+        });
+    for (auto& x : th) x.join();
+    for (int t = 0; t < nt; t++) {
+        if (!err[t].empty()) {
+            std::cerr << "thread " << t << ": " << err[t] << "\n";
+            return 1;
+        }
+        if (bad[t] >= 0) {
+            std::cerr << "concurrent single-query result differs from batch at query " << bad[t] << "\n";
+            return 1;
+        }
+    }
+    return 0;
+}
+
 template <class T>
 static int run_search(char** a) {
     const std::string graph = a[2], base = a[3], qpath = a[4];
@@ -80,7 +117,7 @@ static int run_search(char** a) {
         return 1;
     }
     write_results(a[14], res, k);
-    return compare_single<T>(g, q, nq, k, ef, res);
+    return compare_single<T>(g, q, nq, k, ef, res) || compare_threads<T>(g, q, nq, k, ef, res);
 }
 
 template <class T>
@@ -104,7 +141,7 @@ static int run_build(char** a) {
     auto res = g.AnnSearch<T, std::dynamic_extent>(k, ef, q);
     write_results(a[16], res, k);
     g.Save(a[17]);
-    return compare_single<T>(g, q, nq, k, ef, res);
+    return compare_single<T>(g, q, nq, k, ef, res) || compare_threads<T>(g, q, nq, k, ef, res);
 }
 
 // Behaviour that needs no device: argument and state errors are thrown as

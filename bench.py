@@ -419,9 +419,13 @@ def run_serving(args):
     gq = queries[:1000]
     gt_ids = idx.brute_force(gq)[0][:, :10]
     gt_s = time.perf_counter() - tg0
+    search = idx.search_fast if args.serve_fast else idx.search  # the mode the loop serves with
+    ef_cap = 512 if args.serve_fast else 630
     recall_by_ef = {}
     for ef in (32, 64, 128, 256, 384, 512, 630):
-        ids = idx.search(gq, 10, ef)[0]
+        if ef > ef_cap:
+            continue
+        ids = search(gq, 10, ef)[0]
         recall_by_ef[ef] = round(recall_at_k(ids, gt_ids, 10), 4)
     log(f"recall@10 by ef {recall_by_ef} (ground truth for {len(gq)} queries in {gt_s:.2f}s)")
 
@@ -430,7 +434,7 @@ def run_serving(args):
     ranges = [(16, 256)]
     ef_target = None
     if args.target_recall:
-        lo, hi = 1, 630
+        lo, hi = 1, ef_cap
         for ef in sorted(recall_by_ef):
             if recall_by_ef[ef] >= args.target_recall:
                 hi = ef
@@ -438,7 +442,7 @@ def run_serving(args):
             lo = ef
         while hi - lo > max(4, hi // 32):  # bisect between the measured points
             mid = (lo + hi) // 2
-            r = recall_at_k(idx.search(gq, 10, mid)[0], gt_ids, 10)
+            r = recall_at_k(search(gq, 10, mid)[0], gt_ids, 10)
             recall_by_ef[mid] = round(r, 4)
             lo, hi = (mid, hi) if r < args.target_recall else (lo, mid)
         ef_target = hi
@@ -455,7 +459,8 @@ def run_serving(args):
         ef_mean = (lo + hi) // 2
         for _ in range(2):
             idx.timer_begin()
-            idx.search_device(d_q, 4096, 10, ef_mean, 10, d_i, d_d, d_c, None)
+            (idx.search_fast_device if args.serve_fast else idx.search_device)(d_q, 4096, 10, ef_mean, 10, d_i, d_d,
+                                                                                d_c, None)
             cap_ms = idx.timer_end()
         cap_qps = 4096 / (cap_ms / 1e3)
         rates[(lo, hi)] = (args.load * cap_qps / mean_size, cap_qps, ef_mean)
@@ -475,7 +480,8 @@ def run_serving(args):
         efs = rng.integers(ef_lo, ef_hi + 1, n_req)
         ks = rng.choice([10, 100], n_req)
         efs = np.maximum(efs, ks)
-        srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=max(max_ef, 100))
+        srv = abi.Server(idx, slots=args.slots, slot_queries=4096, max_k=100, max_ef=max(max_ef, 100),
+                         fast=args.serve_fast)
         tickets = []
         done = []
         t0 = time.perf_counter()
@@ -553,6 +559,7 @@ def run_serving(args):
                    "p50_request_latency_ms": first["p50_request_latency_ms"],
                    "p99_request_latency_ms": first["p99_request_latency_ms"],
                    "mixes": mixes, "build_seconds": round(build_s, 2), "rows": rows_desc,
+                   "search": "fast mode (PK_SERVE_FAST)" if args.serve_fast else "exact (reference heap order)",
                    "recall@10_by_ef": recall_by_ef, "ground_truth_seconds": round(gt_s, 3)},
         "roofline": None, "cpu_baseline": None,
     }
@@ -862,6 +869,8 @@ def main():
     ap.add_argument("--insert-share", default="0,0.05",
                     help="comma list of insert shares, one serving run each on the same index")
     ap.add_argument("--load", type=float, default=0.3, help="offered search load as a fraction of measured capacity")
+    ap.add_argument("--serve-fast", action="store_true",
+                    help="serving: launches run the fast mode (recall measured with it; ef <= 512)")
     ap.add_argument("--target-recall", type=float, default=None,
                     help="serving: also run the mixes at the smallest ef whose recall@10 reaches this")
     ap.add_argument("--slots", type=int, default=4)

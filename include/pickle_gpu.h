@@ -244,6 +244,14 @@ int pk_autotune_search(pk_index *h, const void *d_queries, int64_t nq, int32_t k
  * pk_server_wait returns sees all of it.  All calls are thread-safe.  While a server exists
  * the index's layers cannot be replaced; the handle's own searches still run. */
 int pk_server_create(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, pk_server **out);
+/* pk_server_create with flags.  PK_SERVE_FAST: launches run the fast mode
+ * (pk_search_fast_device and the handle's fast tuning; max_ef <= 512).  A
+ * fast-mode query's result depends only on the graph, the query, its k and
+ * ef and the handle's fast tuning (which launches do not change), so it is
+ * still the same alone and in the mix; it is recall-equivalent to the exact search, not bit-exact. */
+#define PK_SERVE_FAST 1
+int pk_server_create_ex(pk_index *h, int slots, int slot_queries, int max_k, int max_ef, int flags,
+                        pk_server **out);
 /* Queues nq queries (host rows, copied at once); results go to the caller's
  * out_ids / out_dists (nq x k) and out_count (nq, may be NULL) when the
  * request completes, so those buffers must outlive the request.  Queries that

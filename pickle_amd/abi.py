@@ -34,10 +34,12 @@ EXPORTS = [
     "pk_index_insert", "pk_device_copy", "pk_index_set_quantizer", "pk_index_set_full_rows",
     "pk_index_copy_graph", "pk_search_rerank_device", "pk_search_rerank", "pk_search_graph_create",
     "pk_search_graph_launch", "pk_search_graph_destroy", "pk_autotune_search", "pk_server_create",
+    "pk_server_create_ex",
     "pk_server_submit", "pk_server_insert", "pk_server_wait", "pk_server_stats", "pk_server_destroy",
 ]
 ERR_STATE, ERR_DEVICE = -4, -7
 BUILD_EXACT_ROWS = 1
+SERVE_FAST = 1  # pk_server_create_ex flag: launches run the fast mode
 BRUTE_K = 32
 
 
@@ -108,6 +110,7 @@ def lib():
             "pk_search_graph_destroy": (C.c_int, [vp]),
             "pk_autotune_search": (C.c_int, [vp, vp, i64, i32, i32, vp, C.POINTER(C.c_float)]),
             "pk_server_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+            "pk_server_create_ex": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
             "pk_server_submit": (C.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, C.POINTER(i64)]),
             "pk_server_insert": (C.c_int, [vp, vp, vp, i64, C.c_int, C.POINTER(i64)]),
             "pk_server_wait": (C.c_int, [vp, i64, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -191,11 +194,13 @@ class Server:
     wait concurrently."""
 
     def __init__(self, index: "Index", slots: int = 4, slot_queries: int = 4096, max_k: int = 100,
-                 max_ef: int = 256):
+                 max_ef: int = 256, fast: bool = False):
+        """fast: launches run the fast mode (PK_SERVE_FAST; max_ef <= 512)."""
         self.index = index
         self.max_k = max_k
         s = C.c_void_p()
-        check(lib().pk_server_create(index.h, slots, slot_queries, max_k, max_ef, C.byref(s)))
+        check(lib().pk_server_create_ex(index.h, slots, slot_queries, max_k, max_ef, SERVE_FAST if fast else 0,
+                                        C.byref(s)))
         self.s = s
         self._keep = {}
 

@@ -19,9 +19,11 @@
 //     reference's order; a larger width trades a few extra expansions for
 //     fewer dependent round trips per query.
 //   * New neighbours that pass the accept rule are sorted with a wave
-//     bitonic network and merged into W with a bitonic merge (the 64
-//     smallest of W's last row and the sorted batch, then a merge of the
-//     whole array), keeping the ef smallest.
+//     bitonic network and merged into W with bitonic merges of only the
+//     tail rows the batch can reach (its smallest key picks 1, 2, 4 or all
+//     rows), keeping the ef smallest; up to six accepted neighbours are
+//     instead inserted one by one with a DPP row shift.  Duplicates (an id
+//     the filter forgot, back while still in W) are removed in registers.
 //   * The visited filter is a hash table in LDS of 16-bit entries: a
 //     bijective hash of the id (an odd multiplier modulo 2^b, b the bits of
 //     the index capacity) names a slot by its high bits and is stored as a
@@ -32,8 +34,10 @@
 //     places both copies side by side and one is dropped (keeping the
 //     expanded mark); if it left W, its distance is at or above W's
 //     farthest and the accept rule rejects it.  Results therefore do not
-//     depend on the table size, only the work does.  No visited state goes
-//     to device memory.
+//     depend on the table size, only the work does (on data without ties;
+//     with ties, which of two equal-distance ids stays can follow the
+//     grouping of the merges, never a returned distance).  No visited state
+//     goes to device memory.
 //
 // Ties between equal distances are ordered by id here, by heap layout in
 // the reference; without ties, width 1 returns the reference's results.
@@ -51,7 +55,7 @@
 namespace pk {
 
 constexpr uint64_t kKeyInf = ~0ull;
-constexpr int kFastCap = 256;  // new neighbours gathered before a flush (distances + merge)
+constexpr int kFastCap = 256;  // most new neighbours gathered before a flush (distances + merge)
 constexpr int kInsertMax = 6;  // accepted neighbours inserted one by one rather than sorted and merged
 
 struct FastArgs {
@@ -75,6 +79,7 @@ struct FastArgs {
     int32_t tag_bits;   // id bits kept in an entry (<= 15), b - slot_bits
     uint32_t id_mask;   // 2^b - 1
     int32_t lds_per_wave;
+    int32_t list_cap;    // new neighbours gathered before a flush (a multiple of 64, <= kFastCap)
     int32_t max_rounds;  // per query; past it the query fails (flag 1)
     int32_t* qcounter;
     int32_t* err;
@@ -441,8 +446,8 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
     uint16_t* vtab = reinterpret_cast<uint16_t*>(my);
     const int nslot = 1 << A.slot_bits;
     int* cl = reinterpret_cast<int*>(vtab + (nslot < 8 ? 8 : nslot));
-    float* cd = reinterpret_cast<float*>(cl + kFastCap);
-    uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kFastCap);
+    float* cd = reinterpret_cast<float*>(cl + A.list_cap);
+    uint8_t* qst = reinterpret_cast<uint8_t*>(cd + A.list_cap);
     const bool leader = lane == 0;
     const int g = lane % DF::LPC;
 
@@ -516,7 +521,7 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
                     const RowHead rh = load_row_head(L, v, lane);
                     l0.neighbors += rh.deg;
                     for (int b0 = 0; b0 < rh.deg; b0 += kWave) {
-                        if (n + kWave > kFastCap) {
+                        if (n + kWave > A.list_cap) {
                             flush(df, L, A.vstride, cl, cd, n, W, ef, len, lane);
                             l0.dists += n;
                             n = 0;
@@ -652,28 +657,10 @@ using namespace pk;
 
 namespace pk {
 int search_class_of(const pk_index* h);
-int ensure_search_counters(pk_index* h);
-}
 
-extern "C" {
-
-int pk_set_fast_tuning(pk_index* h, int width, int hash_bits, int waves_per_group, int blocks_per_cu) {
-    if (!h) return set_error(PK_ERR_ARG, "null handle");
-    if (h->servers_alive) return set_error(PK_ERR_STATE, "destroy the serving loops before changing the tuning");
-    if (width < 0 || width > 4 || (hash_bits != 0 && (hash_bits < 8 || hash_bits > 14)) ||
-        (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4) ||
-        blocks_per_cu < 0)
-        return set_error(PK_ERR_ARG, "bad fast tuning (width 0..4, hash_bits 0 or 8..14, waves_per_group 0/1/2/4)");
-    h->fast.width = width;
-    h->fast.hash_bits = hash_bits;
-    h->fast.waves_per_group = waves_per_group;
-    h->fast.blocks_per_cu = blocks_per_cu;
-    return PK_OK;
-}
-
-int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
-                          int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
-                          float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
+                   const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
+                   int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
     if (!h) return set_error(PK_ERR_ARG, "null handle");
     if (!h->ready) return set_error(PK_ERR_STATE, "index not loaded (call pk_index_set_enter_level)");
     if (nq < 0 || (nq > 0 && (!d_queries || !d_out_ids || !d_out_dists))) return set_error(PK_ERR_ARG, "bad buffers");
@@ -683,7 +670,7 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     if (ef_max < 1 || ef_max > 512) return set_error(PK_ERR_ARG, "fast mode takes ef_max in [1, 512]");
     if (nq > (int64_t)1 << 27) return set_error(PK_ERR_CAPACITY, "at most 2^27 queries per call");
     if (nq == 0) return PK_OK;
-    int rc = ensure_search_counters(h);
+    int rc = ensure_ctx_counters(c);
     if (rc) return rc;
     const FastTuning& t = h->fast;
     const int rr = beam_rows(ef_max);
@@ -697,7 +684,12 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     while (b < 27 && ((int64_t)1 << b) < h->capacity) b++;
     int sb = std::max(hb + 1, b - 15);
     if (b < sb) b = sb;
-    size_t per_wave = ((size_t)2 << std::max(sb, 3)) + (size_t)kFastCap * 8 + h->vstride;
+    // the candidate list holds one round's new neighbours when the rows are
+    // within D: width x D slots rounded to whole chunks of 64 (longer rows
+    // flush part-way through a round)
+    const int D0 = h->layers[0].D > 0 ? h->layers[0].D : 2 * h->M;
+    const int list_cap = std::min(kFastCap, std::max(128, width * ((D0 + 63) / 64) * 64));
+    size_t per_wave = ((size_t)2 << std::max(sb, 3)) + (size_t)list_cap * 8 + h->vstride;
     per_wave = (per_wave + 15) / 16 * 16;
     const size_t lds = per_wave * wpg;
     if (lds > 160 * 1024) return set_error(PK_ERR_CAPACITY, "fast search needs too much LDS per workgroup");
@@ -729,15 +721,43 @@ int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const 
     a.tag_bits = b - sb;
     a.id_mask = b >= 32 ? 0xffffffffu : ((1u << b) - 1u);
     a.lds_per_wave = (int)per_wave;
+    a.list_cap = list_cap;
     a.max_rounds = (int)std::min<int64_t>(1 << 30, 4 * h->capacity + 64);
-    a.qcounter = h->sc.qcounter;
-    a.err = h->sc.err;
+    a.qcounter = c.qcounter;
+    a.err = c.err;
     const hipError_t e =
-        launch_fast(a, h->dtype, h->metric, search_class_of(h), rr, grid, 64 * wpg, lds, h->stream);
+        launch_fast(a, h->dtype, h->metric, search_class_of(h), rr, grid, 64 * wpg, lds, c.stream);
     if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("fast search launch: ") + hipGetErrorString(e));
-    h->last_grid = grid;
-    h->last_lds = lds;
+    if (&c == &h->sc) {
+        h->last_grid = grid;
+        h->last_lds = lds;
+    }
     return PK_OK;
+}
+}  // namespace pk
+
+extern "C" {
+
+int pk_set_fast_tuning(pk_index* h, int width, int hash_bits, int waves_per_group, int blocks_per_cu) {
+    if (!h) return set_error(PK_ERR_ARG, "null handle");
+    if (h->servers_alive) return set_error(PK_ERR_STATE, "destroy the serving loops before changing the tuning");
+    if (width < 0 || width > 4 || (hash_bits != 0 && (hash_bits < 8 || hash_bits > 14)) ||
+        (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4) ||
+        blocks_per_cu < 0)
+        return set_error(PK_ERR_ARG, "bad fast tuning (width 0..4, hash_bits 0 or 8..14, waves_per_group 0/1/2/4)");
+    h->fast.width = width;
+    h->fast.hash_bits = hash_bits;
+    h->fast.waves_per_group = waves_per_group;
+    h->fast.blocks_per_cu = blocks_per_cu;
+    return PK_OK;
+}
+
+int pk_search_fast_device(pk_index* h, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
+                          int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids,
+                          float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters) {
+    if (!h) return set_error(PK_ERR_ARG, "null handle");
+    return fast_search_on(h, h->sc, d_queries, nq, d_k, d_ef, k_all, ef_all, ef_max, k_stride, d_out_ids,
+                          d_out_dists, d_out_count, d_counters);
 }
 
 // Coordinate descent over the fast mode's tuning at one (k, ef) on the

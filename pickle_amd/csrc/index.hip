@@ -566,11 +566,11 @@ static int search_class(const pk_index* h) {
 // the size class a search on h runs with (fast.hip)
 int search_class_of(const pk_index* h) { return search_class(h); }
 
-// the work counter and failure flags of the handle's own search context
-int ensure_search_counters(pk_index* h) {
+// a search context's work counter and failure flags
+int ensure_ctx_counters(SearchCtx& c) {
     int rc;
-    if (!h->sc.qcounter && (rc = dalloc(&h->sc.qcounter, 1))) return rc;
-    if (!h->sc.err && (rc = dalloc(&h->sc.err, 1))) return rc;
+    if (!c.qcounter && (rc = dalloc(&c.qcounter, 1))) return rc;
+    if (!c.err && (rc = dalloc(&c.err, 1))) return rc;
     return PK_OK;
 }
 

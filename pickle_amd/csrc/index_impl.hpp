@@ -111,6 +111,13 @@ hipError_t upload_rows(uint8_t* dst, size_t dst_pitch, const void* src, size_t r
 int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k, const int32_t* d_ef,
               int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride, int32_t* d_out_ids, float* d_out_dists,
               int32_t* d_out_count, pk_query_counters* d_counters, int vbits_override = 0);
+// one fast-mode search (fast.hip) on a given context: its stream and its
+// work counter and failure flags; the handle's own calls use h->sc
+int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
+                   const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
+                   int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters);
+// the context's work counter and failure flags, allocated once
+int ensure_ctx_counters(SearchCtx& c);
 void free_search_ctx(SearchCtx& c);
 }  // namespace pk
 

@@ -95,6 +95,7 @@ struct Slot {
 struct pk_server {
     pk_index* h = nullptr;
     int slot_queries = 0, max_k = 0, max_ef = 0;
+    bool fast = false;  // launches run the fast mode (k_search_fast)
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -161,8 +162,10 @@ int launch_slot(pk_server* S, Slot& s) {
     SCHK(hipMemcpyAsync(s.d_q, s.h_q, nq * rb, hipMemcpyHostToDevice, st));
     SCHK(hipMemcpyAsync(s.d_k, s.h_k, nq * 4, hipMemcpyHostToDevice, st));
     SCHK(hipMemcpyAsync(s.d_ef, s.h_ef, nq * 4, hipMemcpyHostToDevice, st));
-    int rc = search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, ef_launch, S->max_k, s.d_ids, s.d_d, s.d_cnt,
-                       nullptr, s.vbits);
+    int rc = S->fast ? fast_search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, ef_launch, S->max_k, s.d_ids,
+                                      s.d_d, s.d_cnt, nullptr)
+                     : search_on(S->h, s.ctx, s.d_q, nq, s.d_k, s.d_ef, 0, 0, ef_launch, S->max_k, s.d_ids, s.d_d,
+                                 s.d_cnt, nullptr, s.vbits);
     if (rc) return rc;
     SCHK(hipMemcpyAsync(s.h_ids, s.d_ids, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
     SCHK(hipMemcpyAsync(s.h_d, s.d_d, nq * S->max_k * 4, hipMemcpyDeviceToHost, st));
@@ -229,7 +232,9 @@ void complete_slot(pk_server* S, Slot& s, int status) {
             std::vector<int64_t> bad;
             for (int64_t i = 0; i < r->nq; i++)
                 if (s.h_cnt[f + i] < 0) bad.push_back(i);
-            if (!bad.empty() && s.ctx.last_vbits < 20) {
+            // (the fast mode keeps no visited table: its queries fail only
+            // on arguments, which submit checked, so they are not rerun)
+            if (!bad.empty() && !S->fast && s.ctx.last_vbits < 20) {
                 Request* c = r;  // a rerun that failed again runs again, whole
                 if (!r->parent) {
                     c = new Request;
@@ -374,13 +379,20 @@ void insert_worker(pk_server* S) {
 extern "C" {
 
 int pk_server_create(pk_index* h, int slots, int slot_queries, int max_k, int max_ef, pk_server** out) {
+    return pk_server_create_ex(h, slots, slot_queries, max_k, max_ef, 0, out);
+}
+
+int pk_server_create_ex(pk_index* h, int slots, int slot_queries, int max_k, int max_ef, int flags, pk_server** out) {
     if (!h || !out) return set_error(PK_ERR_ARG, "null argument");
     *out = nullptr;
     if (!h->ready) return set_error(PK_ERR_STATE, "index not loaded");
     if (slots < 1 || slots > 8 || slot_queries < 1 || max_k < 1 || max_ef < max_k || max_ef > 630)
         return set_error(PK_ERR_ARG, "need 1..8 slots, slot_queries >= 1, 1 <= max_k <= max_ef <= 630");
+    if (flags & ~PK_SERVE_FAST) return set_error(PK_ERR_ARG, "unknown serving flags");
+    if ((flags & PK_SERVE_FAST) && max_ef > 512) return set_error(PK_ERR_ARG, "the fast mode takes max_ef <= 512");
     auto* S = new pk_server;
     S->h = h;
+    S->fast = (flags & PK_SERVE_FAST) != 0;
     S->slot_queries = slot_queries;
     S->max_k = max_k;
     S->max_ef = max_ef;

@@ -94,6 +94,29 @@ def test_results_independent_of_filter_size(graphs, name):
     idx.close()
 
 
+@pytest.mark.parametrize("name", ["f32_ip", "f16_l2"])
+def test_results_independent_of_filter_tags(graphs, name):
+    """The filter keeps b - slot_bits bits of each hashed id per entry, b
+    the bits of the index capacity: none for a 3,000-row index, up to 15
+    for a capacity of 2^24 + 1 (25-bit ids).  The same graph searched
+    through both returns the same results at every filter size (tie-free
+    data)."""
+    fx, p, base, queries, g = graphs(name)
+    small = index_from_oracle(g)
+    big = index_from_oracle(g, capacity=(1 << 24) + 1)
+    for hb in (8, 11, 14):
+        say("case", name, "hash_bits", hb)
+        small.set_fast_tuning(width=2, hash_bits=hb)
+        big.set_fast_tuning(width=2, hash_bits=hb)
+        a = small.search_fast(queries, 10, 64)
+        b = big.search_fast(queries, 10, 64)
+        np.testing.assert_array_equal(b[2], a[2])
+        np.testing.assert_array_equal(b[0], a[0])
+        np.testing.assert_array_equal(b[1], a[1])
+    small.close()
+    big.close()
+
+
 @pytest.mark.parametrize("name", G.GRAPH_CASES)
 def test_ef_covering_graph_is_brute_force(name):
     """A 400-node oracle graph of each golden case's first rows, searched

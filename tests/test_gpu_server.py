@@ -120,3 +120,48 @@ def test_request_that_overflows_scratch_reruns_alone():
     assert e.value.code == -1
     srv.close()
     idx.close()
+
+
+def test_fast_mode_requests_alone_equal_mixed():
+    """PK_SERVE_FAST: launches run the fast mode with per-query k and ef and
+    the launch's largest ef; each request equals the same queries searched
+    alone with pk_search_fast_device (a query's result depends only on the
+    graph, the query, k, ef and the handle's fast tuning)."""
+    n, dim, M, efc = 20000, 64, 12, 80
+    base, queries = datasets.gaussian_mixture(n, 2000, dim, 32, 43, kind="u8")
+    idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
+    idx.build(base, max_batch=4096, seed=43)
+    rng = np.random.default_rng(7)
+    reqs, at = [], 0
+    while at < len(queries):
+        m = int(min(len(queries) - at, np.exp(rng.uniform(0, np.log(300)))))
+        reqs.append((at, m, int(rng.choice([1, 10, 32])), int(rng.integers(32, 300))))
+        at += m
+    say("requests", len(reqs))
+    alone = [idx.search_fast(queries[s:s + m], k, ef) for s, m, k, ef in reqs]
+    srv = abi.Server(idx, slots=4, slot_queries=512, max_k=32, max_ef=300, fast=True)
+    results = [None] * len(reqs)
+
+    def client(part):
+        tickets = [(i, srv.submit(queries[reqs[i][0]:reqs[i][0] + reqs[i][1]], reqs[i][2], reqs[i][3]))
+                   for i in part]
+        for i, t in tickets:
+            results[i] = srv.wait(t)
+
+    threads = [threading.Thread(target=client, args=(list(range(j, len(reqs), 4)),)) for j in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    st = srv.stats()
+    say("stats", st)
+    srv.close()
+    for i, (s, m, k, ef) in enumerate(reqs):
+        ids, ds, cnt, lat, que = results[i]
+        np.testing.assert_array_equal(cnt, alone[i][2], err_msg=f"request {i}")
+        np.testing.assert_array_equal(ids, alone[i][0][:, :k], err_msg=f"request {i}")
+        np.testing.assert_array_equal(ds, alone[i][1][:, :k], err_msg=f"request {i}")
+    assert st["queries"] == len(queries) and st["launches"] < len(reqs)
+    with pytest.raises(abi.PickleError):
+        abi.Server(idx, max_k=10, max_ef=600, fast=True)  # the fast mode's ef limit
+    idx.close()

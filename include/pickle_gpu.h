@@ -171,6 +171,14 @@ int pk_search_fast_device(pk_index *h, const void *d_queries, int64_t nq, const 
                           int32_t k_stride, int32_t *d_out_ids, float *d_out_dists,
                           int32_t *d_out_count, pk_query_counters *d_counters);
 
+/* pk_search_fast_device with host buffers, as pk_search (copies in, runs on
+ * the handle's stream, copies out; synchronous); ef <= 512.  The caller
+ * AnnSearch(k, ef, NDArray) (include/hnsw.hpp:528-541) in fast mode.
+ * PK_ERR_DEVICE if a query failed on the device (flags in the message). */
+int pk_search_fast(pk_index *h, const void *queries, int64_t nq, const int32_t *k, const int32_t *ef,
+                   int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t *out_ids, float *out_dists,
+                   int32_t *out_count, pk_query_counters *counters);
+
 /* Fast-mode tuning (0 = automatic for each): width 1..4 entries expanded
  * per round (default 2), hash_bits 8..14 (each query's LDS visited filter
  * takes 4 << hash_bits bytes: 2^(hash_bits + 1) 16-bit entries; default 11), waves_per_group 1, 2 or 4 (queries per

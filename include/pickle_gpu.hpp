@@ -209,28 +209,25 @@ class HNSWGraph {
         check(pk_index_insert(m_h.get(), ext_ids.data(), (int64_t)ext_ids.size(), m_batch, m_flags));
     }
 
-    /* include/hnsw.hpp:509-525.  Without metrics (and with k <= ef <= 630)
-     * the query goes through the graph's serving loop, so calls from many
-     * threads at once run as batched launches; with metrics it runs alone on
-     * the handle's stream, whose counters LastCounters() reports. */
+    /* include/hnsw.hpp:509-525.  Without metrics (and with k <= ef <= 630,
+     * 512 in fast mode) the query goes through the graph's serving loop, so
+     * calls from many threads at once run as batched launches; with metrics
+     * it runs alone on the handle's stream, whose counters LastCounters()
+     * reports. */
     template <class T, size_t Dim, bool collect_metric = false>
     std::vector<Entry> AnnSearch(int top_k, int search_ef, std::span<const T, Dim> query) const {
         if (top_k < 1 || search_ef < 1) throw Error(PK_ERR_ARG, "k and ef must be positive");
-        if (collect_metric || search_ef < top_k || search_ef > kServeMaxEf) {
-            std::shared_lock<std::shared_mutex> g(m_mu);
-            std::lock_guard<std::mutex> d(m_direct);
-            ready();
-            check_type<T>(m_dtype);
-            if (query.size() != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
-            std::vector<std::vector<Entry>> r = search(top_k, search_ef, query.data(), 1, collect_metric);
-            return std::move(r[0]);
-        }
         for (;;) {
             {
                 std::shared_lock<std::shared_mutex> g(m_mu);
                 ready();
                 check_type<T>(m_dtype);
                 if (query.size() != m_dim) throw Error(PK_ERR_ARG, "query width != index dim");
+                if (collect_metric || search_ef < top_k || search_ef > serve_max_ef()) {
+                    std::lock_guard<std::mutex> d(m_direct);
+                    std::vector<std::vector<Entry>> r = search(top_k, search_ef, query.data(), 1, collect_metric);
+                    return std::move(r[0]);
+                }
                 if (m_server && top_k <= m_serve_k) return serve_one(top_k, search_ef, query.data());
             }
             // no loop yet, or one whose result rows are narrower than k
@@ -238,6 +235,18 @@ class HNSWGraph {
             ready();
             if (!m_server || top_k > m_serve_k) start_server(top_k);
         }
+    }
+
+    /* Fast mode (no counterpart in the reference): AnnSearch runs the fast
+     * search (pk_search_fast, and a PK_SERVE_FAST loop for concurrent
+     * single-query calls), whose results are recall-equivalent to the
+     * reference's, not bit-exact; ef above 512 keeps the exact search.  Off
+     * by default. */
+    void set_fast_search(bool on) {
+        std::unique_lock<std::shared_mutex> g(m_mu);
+        if (on == m_fast) return;
+        retire_server();
+        m_fast = on;
     }
 
     /* include/hnsw.hpp:528-541 */
@@ -327,11 +336,13 @@ class HNSWGraph {
     }
     // the serving loop behind concurrent AnnSearch(span) calls: result rows
     // of `k` entries (at least 100), any ef in [k, 630]
+    int serve_max_ef() const { return m_fast ? kFastMaxEf : kServeMaxEf; }
     void start_server(int k) const {
         retire_server();
-        const int kk = std::min(kServeMaxEf, std::max({k, 100, m_serve_k}));
+        const int kk = std::min(serve_max_ef(), std::max({k, 100, m_serve_k}));
         pk_server* s = nullptr;
-        check(pk_server_create(m_h.get(), kServeSlots, kServeQueries, kk, kServeMaxEf, &s));
+        check(pk_server_create_ex(m_h.get(), kServeSlots, kServeQueries, kk, serve_max_ef(),
+                                  m_fast ? PK_SERVE_FAST : 0, &s));
         m_server.reset(s);
         m_serve_k = kk;
     }
@@ -358,6 +369,7 @@ class HNSWGraph {
         void operator()(pk_server* s) const { (void)pk_server_destroy(s); }
     };
     static constexpr int kServeMaxEf = 630;  // the largest ef a search supports
+    static constexpr int kFastMaxEf = 512;   // the largest ef of the fast mode
     static constexpr int kServeSlots = 4;
     static constexpr int kServeQueries = 1024;
 
@@ -366,8 +378,9 @@ class HNSWGraph {
         std::vector<int32_t> ids(nq * k), cnt(nq);
         std::vector<float> ds(nq * k);
         if (collect) m_counters.assign(nq, pk_query_counters{});
-        check(pk_search(m_h.get(), q, (int64_t)nq, nullptr, nullptr, k, ef, k, ids.data(), ds.data(), cnt.data(),
-                        collect ? m_counters.data() : nullptr));
+        auto run = m_fast && ef <= kFastMaxEf ? pk_search_fast : pk_search;
+        check(run(m_h.get(), q, (int64_t)nq, nullptr, nullptr, k, ef, k, ids.data(), ds.data(), cnt.data(),
+                  collect ? m_counters.data() : nullptr));
         std::vector<std::vector<Entry>> out(nq);
         for (size_t i = 0; i < nq; i++) {
             if (cnt[i] < 0) throw Error(PK_ERR_DEVICE, "query " + std::to_string(i) + " failed on the device");
@@ -392,6 +405,7 @@ class HNSWGraph {
     // declared after m_h: retired before the index it searches
     mutable std::unique_ptr<pk_server, ServerFree> m_server;
     mutable int m_serve_k{0};
+    bool m_fast{false};
 };
 
 }  // namespace pickle_gpu

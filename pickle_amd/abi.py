@@ -27,7 +27,7 @@ EXPORTS = [
     "pk_last_error", "pk_version", "pk_index_create", "pk_index_destroy", "pk_index_set_base",
     "pk_index_load_layer", "pk_index_set_enter_level", "pk_index_load_file", "pk_index_save_file",
     "pk_index_info", "pk_index_read_layer", "pk_search", "pk_search_device", "pk_search_device_ex",
-    "pk_search_device_static", "pk_search_fast_device", "pk_set_fast_tuning", "pk_search_rerank_fast_device",
+    "pk_search_device_static", "pk_search_fast_device", "pk_search_fast", "pk_set_fast_tuning", "pk_search_rerank_fast_device",
     "pk_autotune_fast", "pk_set_build_row_limit",
     "pk_device_error", "pk_set_tuning", "pk_last_launch", "pk_sync", "pk_device_alloc", "pk_device_free",
     "pk_copy_to_device", "pk_copy_to_host", "pk_timer_begin", "pk_timer_end", "pk_brute_force", "pk_wall_clock_khz", "pk_sample_levels", "pk_index_init_build",
@@ -78,6 +78,7 @@ def lib():
             "pk_search_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device_ex": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_device_static": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+            "pk_search_fast": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp, vp]),
             "pk_search_fast_device": (C.c_int, [vp, vp, i64, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
             "pk_set_fast_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "pk_search_rerank_fast_device": (C.c_int, [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]),
@@ -438,8 +439,8 @@ class Index:
         return out
 
     def search_fast(self, queries: np.ndarray, k, ef, k_stride: int | None = None, counters: bool = False):
-        """Fast-mode search of host queries: (ids, dists, counts[, counters]).
-        k / ef may be ints or per-query arrays."""
+        """Fast-mode search of host queries through pk_search_fast:
+        (ids, dists, counts[, counters]); k / ef may be ints or per-query arrays."""
         q = np.ascontiguousarray(queries, dtype=self.dtype)
         if q.ndim == 1:
             q = q[None, :]
@@ -448,24 +449,13 @@ class Index:
         ef_arr = None if np.isscalar(ef) else np.ascontiguousarray(ef, np.int32)
         if k_stride is None:
             k_stride = int(k) if k_arr is None else int(k_arr.max())
-        ef_max = int(ef) if ef_arr is None else int(ef_arr.max())
-        bufs = [DeviceBuffer.from_array(q), DeviceBuffer(nq * k_stride * 4), DeviceBuffer(nq * k_stride * 4),
-                DeviceBuffer(nq * 4), DeviceBuffer(nq * COUNTER_DTYPE.itemsize) if counters else None,
-                DeviceBuffer.from_array(k_arr) if k_arr is not None else None,
-                DeviceBuffer.from_array(ef_arr) if ef_arr is not None else None]
-        d_q, d_i, d_d, d_c, d_t, d_k, d_e = bufs
-        try:
-            self.search_fast_device(d_q, nq, 0 if k_arr is not None else int(k), 0 if ef_arr is not None else int(ef),
-                                    k_stride, d_i, d_d, d_c, d_t, d_k, d_e, ef_max)
-            self.sync()
-            ids = d_i.to_array(np.int32, (nq, k_stride))
-            ds = d_d.to_array(np.float32, (nq, k_stride))
-            cnt = d_c.to_array(np.int32, (nq,))
-            ctr = d_t.to_array(COUNTER_DTYPE, (nq,)) if counters else None
-        finally:
-            for b in bufs:
-                if b is not None:
-                    b.free()
+        ids = np.empty((nq, k_stride), np.int32)
+        ds = np.empty((nq, k_stride), np.float32)
+        cnt = np.empty(nq, np.int32)
+        ctr = np.zeros(nq, COUNTER_DTYPE) if counters else None
+        check(lib().pk_search_fast(self.h, _p(q), nq, _p(k_arr), _p(ef_arr), 0 if k_arr is not None else int(k),
+                                   0 if ef_arr is not None else int(ef), k_stride, _p(ids), _p(ds), _p(cnt),
+                                   _p(ctr)))
         if counters:
             return ids, ds, cnt, ctr
         return ids, ds, cnt

@@ -964,6 +964,51 @@ static int io_buf(pk_index* h, int slot, size_t bytes, void** p) {
     return PK_OK;
 }
 
+int pk_search_fast(pk_index* h, const void* queries, int64_t nq, const int32_t* k, const int32_t* ef, int32_t k_all,
+                   int32_t ef_all, int32_t k_stride, int32_t* out_ids, float* out_dists, int32_t* out_count,
+                   pk_query_counters* counters) {
+    if (!h) return fail(PK_ERR_ARG, "null handle");
+    if (nq < 0 || (nq > 0 && (!queries || !out_ids || !out_dists))) return fail(PK_ERR_ARG, "bad buffers");
+    if (nq == 0) return PK_OK;
+    int ef_max = ef_all;
+    if (ef) {
+        ef_max = 1;
+        for (int64_t i = 0; i < nq; i++) {
+            if (ef[i] < 1) return fail(PK_ERR_ARG, "ef[%lld] < 1", (long long)i);
+            ef_max = std::max(ef_max, ef[i]);
+        }
+    }
+    if (k)
+        for (int64_t i = 0; i < nq; i++)
+            if (k[i] < 1 || k[i] > k_stride) return fail(PK_ERR_ARG, "k[%lld] outside [1, k_stride]", (long long)i);
+    void *dq, *dk = nullptr, *def = nullptr, *dids, *dd, *dc, *dctr = nullptr;
+    int rc;
+    if ((rc = io_buf(h, 0, (size_t)nq * h->qbytes, &dq)) || (rc = io_buf(h, 3, (size_t)nq * k_stride * 4, &dids)) ||
+        (rc = io_buf(h, 4, (size_t)nq * k_stride * 4, &dd)) || (rc = io_buf(h, 5, (size_t)nq * 4, &dc)))
+        return rc;
+    if (k && (rc = io_buf(h, 1, (size_t)nq * 4, &dk))) return rc;
+    if (ef && (rc = io_buf(h, 2, (size_t)nq * 4, &def))) return rc;
+    if (counters && (rc = io_buf(h, 6, (size_t)nq * sizeof(pk_query_counters), &dctr))) return rc;
+    HIPCHK(hipMemcpyAsync(dq, queries, (size_t)nq * h->qbytes, hipMemcpyHostToDevice, h->stream));
+    if (k) HIPCHK(hipMemcpyAsync(dk, k, (size_t)nq * 4, hipMemcpyHostToDevice, h->stream));
+    if (ef) HIPCHK(hipMemcpyAsync(def, ef, (size_t)nq * 4, hipMemcpyHostToDevice, h->stream));
+    rc = fast_search_on(h, h->sc, dq, nq, (const int32_t*)dk, (const int32_t*)def, k_all, ef_all, ef_max, k_stride,
+                        (int32_t*)dids, (float*)dd, (int32_t*)dc, (pk_query_counters*)dctr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out_ids, dids, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(out_dists, dd, (size_t)nq * k_stride * 4, hipMemcpyDeviceToHost, h->stream));
+    if (out_count) HIPCHK(hipMemcpyAsync(out_count, dc, (size_t)nq * 4, hipMemcpyDeviceToHost, h->stream));
+    if (counters)
+        HIPCHK(hipMemcpyAsync(counters, dctr, (size_t)nq * sizeof(pk_query_counters), hipMemcpyDeviceToHost, h->stream));
+    int32_t errflag = 0;
+    HIPCHK(hipMemcpyAsync(&errflag, h->sc.err, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the fast mode keeps no visited table that could fill: a flag here
+    // means bad per-query arguments or a query past its round limit
+    if (errflag) return fail(PK_ERR_DEVICE, "fast search failed on the device (flags %d)", errflag);
+    return PK_OK;
+}
+
 int pk_search(pk_index* h, const void* queries, int64_t nq, const int32_t* k, const int32_t* ef, int32_t k_all,
               int32_t ef_all, int32_t k_stride, int32_t* out_ids, float* out_dists, int32_t* out_count,
               pk_query_counters* counters) {

@@ -35,6 +35,7 @@ class HNSWIndex:
         self._data: np.ndarray | None = None
         self._index: abi.Index | None = None
         self._index_params = None
+        self._fast = False
 
     # properties (pybind/hnsw.cpp:31-45, 118-123)
     @staticmethod
@@ -75,6 +76,17 @@ class HNSWIndex:
     @M.setter
     def M(self, v):
         self._M = self._positive("M", v)
+
+    @property
+    def fast_search(self) -> bool:
+        """Search with the fast mode (pk_search_fast: recall-equivalent to
+        the reference's search, not bit-exact; search_ef <= 512).  Off by
+        default; no counterpart in the reference module."""
+        return self._fast
+
+    @fast_search.setter
+    def fast_search(self, v: bool):
+        self._fast = bool(v)
 
     @property
     def num_layer(self) -> int:
@@ -121,7 +133,8 @@ class HNSWIndex:
             raise ValueError(f"query width {q.shape[1]} != data width {self._data.shape[1]}")
         if q.dtype != self._data.dtype:
             raise ValueError(f"query dtype {q.dtype} != data dtype {self._data.dtype}")
-        ids, _, cnt = self._index.search(np.ascontiguousarray(q), self._top_k, self._search_ef)
+        run = self._index.search_fast if self._fast else self._index.search
+        ids, _, cnt = run(np.ascontiguousarray(q), self._top_k, self._search_ef)
         if (cnt != self._top_k).any():
             raise RuntimeError(f"{int((cnt != self._top_k).sum())} queries returned fewer than top_k results")
         return ids

@@ -100,6 +100,34 @@ static int compare_threads(const HNSWGraph& g, const NDArray& q, size_t nq, int 
     return 0;
 }
 
+// fast mode: concurrent single-query calls equal its batch, and the batch
+// finds nearly the same neighbours as the exact search
+template <class T>
+static int check_fast(HNSWGraph& g, const NDArray& q, size_t nq, int k, int ef,
+                      const std::vector<std::vector<Entry>>& exact) {
+    if (ef > 512) return 0;
+    g.set_fast_search(true);
+    auto fres = g.AnnSearch<T, std::dynamic_extent>(k, ef, q);
+    if (compare_threads<T>(g, q, nq, k, ef, fres)) return 1;
+    size_t same = 0, total = 0;
+    for (size_t i = 0; i < nq; i++) {
+        for (const Entry& e : fres[i]) {
+            for (const Entry& x : exact[i])
+                if (x.m_vid == e.m_vid) {
+                    same++;
+                    break;
+                }
+        }
+        total += exact[i].size();
+    }
+    g.set_fast_search(false);
+    if (total && same < 0.9 * total) {
+        std::cerr << "fast mode shares only " << same << " of " << total << " ids with the exact search\n";
+        return 1;
+    }
+    return 0;
+}
+
 template <class T>
 static int run_search(char** a) {
     const std::string graph = a[2], base = a[3], qpath = a[4];
@@ -117,7 +145,8 @@ static int run_search(char** a) {
         return 1;
     }
     write_results(a[14], res, k);
-    return compare_single<T>(g, q, nq, k, ef, res) || compare_threads<T>(g, q, nq, k, ef, res);
+    if (compare_single<T>(g, q, nq, k, ef, res) || compare_threads<T>(g, q, nq, k, ef, res)) return 1;
+    return check_fast<T>(g, q, nq, k, ef, res);
 }
 
 template <class T>

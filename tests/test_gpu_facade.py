@@ -112,3 +112,9 @@ def test_pyann_build_search_save_load(tmp_path):
     np.testing.assert_array_equal(idx2.search(queries), ids)
     with pytest.raises(ValueError):
         idx2.search(queries.astype(np.int8))
+    # the fast mode: recall-equivalent to the exact search
+    idx2.fast_search = True
+    fids = idx2.search(queries)
+    frec = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(fids, gt)])
+    say("  fast-mode recall@10", frec)
+    assert frec >= rec - 0.01

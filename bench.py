@@ -842,6 +842,8 @@ def main():
     ap.add_argument("--build", choices=["cpu", "gpu"], default=None,
                     help="graph from the reference's CPU build or the GPU batched build (default per workload)")
     ap.add_argument("--insert-batch", type=int, default=16384)
+    ap.add_argument("--build-graph", action="store_true",
+                    help="GPU builds run each insert batch's kernel phases as HIP graphs (PK_BUILD_GRAPH)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", default=None,
                     help="comma list of low-precision traversal copies (f16, i8) searched + FP32 re-ranked "
@@ -986,9 +988,11 @@ def main():
         idx.set_row_limit(row_limit)
         tb = time.perf_counter()
         idx.build(base, max_batch=args.insert_batch, seed=args.seed, exact_rows=not args.capped_rows,
+                  graph=args.build_graph,
                   progress=lambda d, t: log(f"  inserted {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
         idx.sync()
         build_info = {"build_seconds": time.perf_counter() - tb, "threads": 0, "where": "gpu",
+                      "hip_graphs": bool(args.build_graph),
                       "insert_batch": args.insert_batch, "capped_rows": bool(args.capped_rows), "efc": efc,
                       "row_limit": row_limit or 511}
         log(f"GPU build in {build_info['build_seconds']:.1f}s")

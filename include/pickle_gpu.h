@@ -317,6 +317,13 @@ int pk_wall_clock_khz(int *khz);
  * max_batch = 1 the build equals the reference's single-threaded Build.
  * Without it rows never exceed D entries. */
 #define PK_BUILD_EXACT_ROWS 1
+/* pk_index_insert flag (no counterpart in the reference): each batch's two
+ * kernel phases (the construction searches; AddNode + AddEdge) run as HIP
+ * graphs, captured from the launch sequence, instantiated once per index and
+ * updated in place for every later batch.  Results are those of the stream
+ * launches.  While it runs, nothing else may be enqueued on the index's own
+ * stream (its own searches), as for any insertion beside searches. */
+#define PK_BUILD_GRAPH 2
 
 /* Level sampler of include/util.hpp:13-64: a fresh uniform per level,
  * P(level >= l) = M^(-l(l+1)/2); seeded (the reference seeds from

@@ -39,6 +39,7 @@ EXPORTS = [
 ]
 ERR_STATE, ERR_DEVICE = -4, -7
 BUILD_EXACT_ROWS = 1
+BUILD_GRAPH = 2  # pk_index_insert flag: batches run as HIP graphs
 SERVE_FAST = 1  # pk_server_create_ex flag: launches run the fast mode
 BRUTE_K = 32
 
@@ -471,13 +472,15 @@ class Index:
             raise ValueError("levels must cover the whole capacity")
         check(lib().pk_index_init_build(self.h, _p(lv)))
 
-    def insert(self, ext_ids, max_batch: int = 4096, exact_rows: bool = True) -> None:
+    def insert(self, ext_ids, max_batch: int = 4096, exact_rows: bool = True, graph: bool = False) -> None:
+        """graph: each batch's kernel phases run as HIP graphs (PK_BUILD_GRAPH)."""
         ids = np.ascontiguousarray(ext_ids, np.int32)
-        check(lib().pk_index_insert(self.h, _p(ids), ids.shape[0], max_batch,
-                                    BUILD_EXACT_ROWS if exact_rows else 0))
+        flags = (BUILD_EXACT_ROWS if exact_rows else 0) | (BUILD_GRAPH if graph else 0)
+        check(lib().pk_index_insert(self.h, _p(ids), ids.shape[0], max_batch, flags))
 
     def build(self, base: np.ndarray, levels: np.ndarray | None = None, order=None, max_batch: int = 4096,
-              exact_rows: bool = True, seed: int = 0, progress=None, chunk: int = 1 << 20) -> None:
+              exact_rows: bool = True, seed: int = 0, progress=None, chunk: int = 1 << 20,
+              graph: bool = False) -> None:
         """Build (include/hnsw.hpp:497-507): base rows, levels, inserts.  With
         `progress`, ids are inserted in chunks and progress(done, total) is
         called after each."""
@@ -488,10 +491,10 @@ class Index:
         if order is None:
             order = np.arange(base.shape[0], dtype=np.int32)
         if progress is None:
-            self.insert(order, max_batch=max_batch, exact_rows=exact_rows)
+            self.insert(order, max_batch=max_batch, exact_rows=exact_rows, graph=graph)
             return
         for s in range(0, len(order), chunk):
-            self.insert(order[s:s + chunk], max_batch=max_batch, exact_rows=exact_rows)
+            self.insert(order[s:s + chunk], max_batch=max_batch, exact_rows=exact_rows, graph=graph)
             progress(min(len(order), s + chunk), len(order))
 
     def brute_force(self, queries: np.ndarray):

@@ -292,6 +292,8 @@ int pk_index_destroy(pk_index* h) {
     dfree(h->ins.vlog);
     dfree(h->ins.counter);
     dfree(h->ins.err);
+    for (auto& ex : h->ins.graph)
+        if (ex) (void)hipGraphExecDestroy(ex);
     for (auto& b : h->io) dfree(b.p);
     for (auto& b : h->rr) dfree(b.p);
     dfree(h->full);

@@ -48,6 +48,9 @@ struct InsertScratch {
     int vbits = 0;          // visited-table size the last batch needed
     int32_t* counter = nullptr;
     int32_t* err = nullptr;
+    // PK_BUILD_GRAPH: the executable graph of each insert phase (0: the
+    // construction searches, 1: AddNode + AddEdge), updated batch by batch
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 
 struct SearchTuning {

@@ -827,8 +827,51 @@ extern "C" int pk_index_init_build(pk_index* h, const uint8_t* levels) {
     return pk::publish_layers(h);
 }
 
+// One insert phase's kernels on the index stream, directly or (graph) as a
+// HIP graph: the launch sequence is captured, and the phase's executable
+// graph is instantiated on first use and updated in place with the new
+// arguments and grid afterwards (instantiated again if an update is refused).
+static int run_insert_phase(pk_index* h, const InsertArgs& a, int phase, int grid, size_t lds_a, size_t lds_c,
+                            bool graph) {
+    if (!graph) {
+        ICHK(launch_insert(a, phase, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+        return PK_OK;
+    }
+    hipGraph_t g = nullptr;
+    ICHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t le = launch_insert(a, phase, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream);
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_error(PK_ERR_HIP, std::string("insert graph capture: ") +
+                                         hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipGraphExec_t& ex = h->ins.graph[phase];
+    if (ex) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(ex, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();  // the refused update leaves no error behind
+            (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (!ex) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ex, h->stream);
+    (void)hipGraphDestroy(g);  // the executable graph holds its own copy
+    if (e != hipSuccess) {
+        if (ex) (void)hipGraphExecDestroy(ex);
+        ex = nullptr;
+        return set_error(PK_ERR_HIP, std::string("insert graph: ") + hipGetErrorString(e));
+    }
+    return PK_OK;
+}
+
 extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, int max_batch, int flags) {
     if (!h || (n > 0 && !ext_ids)) return set_error(PK_ERR_ARG, "null argument");
+    if (flags & ~(PK_BUILD_EXACT_ROWS | PK_BUILD_GRAPH)) return set_error(PK_ERR_ARG, "unknown insert flags");
+    const bool graph = (flags & PK_BUILD_GRAPH) != 0;
     if (!h->building) return set_error(PK_ERR_STATE, "call pk_index_init_build first");
     if (max_batch < 1) max_batch = 4096;
     for (int64_t t = 0; t < n; t++)
@@ -967,7 +1010,7 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
         a.err = h->ins.err;
         int grid = (int)std::min<int64_t>({nb, (int64_t)grid_cap, (int64_t)pk::insert_scratch_waves(h, vbits)});
         for (;;) {
-            ICHK(launch_insert(a, 0, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+            if ((rc = run_insert_phase(h, a, 0, grid, lds_a, lds_c, graph))) return rc;
             int32_t err = 0;
             if ((rc = device_flags(&err))) return rc;
             if (err == 0) break;
@@ -985,7 +1028,7 @@ extern "C" int pk_index_insert(pk_index* h, const int32_t* ext_ids, int64_t n, i
             a.vg_bits = vbits;
             ICHK(hipMemsetAsync(h->ins.err, 0, 32, h->stream));
         }
-        ICHK(launch_insert(a, 1, h->dtype, h->metric, h->size_class, grid, lds_a, lds_c, h->stream));
+        if ((rc = run_insert_phase(h, a, 1, grid, lds_a, lds_c, graph))) return rc;
         {
             int32_t err = 0;
             if ((rc = device_flags(&err))) return rc;

@@ -22,13 +22,17 @@ def say(*a):
     sys.stdout.flush()
 
 
+@pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("name", ["u8_l2", "i8_ip", "f32_l2", "f16_l2"])
-def test_exact_build_matches_reference(name):
+def test_exact_build_matches_reference(name, graph):
+    """Batch-1 builds with reference rows equal the reference's build.  With
+    graph, every batch's two phases replay HIP graphs updated in place
+    (thousands of replays, host work and a sync between them)."""
     fx, p, base, queries = G.graph_case(name)
-    say("exact build", name, p)
+    say("exact build", name, "graph" if graph else "stream", p)
     g = G.oracle_graph(fx, p, base)  # equal to the reference's build (test_oracle_golden)
     idx = abi.Index(base.dtype, abi.L2 if p["metric"] == "l2" else abi.IP, p["dim"], p["M"], p["efc"], p["n"])
-    idx.build(base, levels=fx["levels"], max_batch=1, exact_rows=True)
+    idx.build(base, levels=fx["levels"], max_batch=1, exact_rows=True, graph=graph)
     info = idx.info()
     assert info["enter_level"] == int(fx["enter_level"])
     for l in range(g.nlayers):
@@ -74,10 +78,11 @@ def test_batched_build_recall_close_to_cpu_graph(reference_rows):
     gt, _ = O.brute_force(base, queries, 10, O.L2)
     # two batched builds: the order of concurrent AddEdge calls differs from
     # build to build, and their mean recall halves that variance
+    # (the second runs its batches as HIP graphs, PK_BUILD_GRAPH)
     builds = []
-    for _ in range(2):
+    for b in range(2):
         idx = abi.Index(np.uint8, abi.L2, dim, M, efc, n)
-        idx.build(base, levels=levels, max_batch=4096, exact_rows=reference_rows)
+        idx.build(base, levels=levels, max_batch=4096, exact_rows=reference_rows, graph=b == 1)
         if not reference_rows:
             for l in range(idx.info()["nlayers"]):
                 L = idx.read_layer(l)

@@ -389,7 +389,7 @@ def run_serving(args):
                     idx.init_build(levels)
                     started = True
                 idx.insert(np.arange(at, at + nb, dtype=np.int32), max_batch=args.insert_batch,
-                           exact_rows=not args.capped_rows)
+                           exact_rows=not args.capped_rows, graph=not args.stream_build)
             if nb < m:
                 tail.append(c[nb:])
             at += m
@@ -403,7 +403,7 @@ def run_serving(args):
         chunk = 1 << 20
         for s0 in range(0, n, chunk):
             idx.insert(np.arange(s0, min(n, s0 + chunk), dtype=np.int32), max_batch=args.insert_batch,
-                       exact_rows=not args.capped_rows)
+                       exact_rows=not args.capped_rows, graph=not args.stream_build)
             if n > chunk:
                 log(f"  inserted {min(n, s0 + chunk):,} / {n:,} ({time.perf_counter() - tb:.1f}s)")
         extra_rows = base[n:]
@@ -607,7 +607,7 @@ def run_uneven(args):
     # graphs: GPU batched build and the reference's CPU build
     idx = abi.Index(base.dtype, mcode, dim, M, efc, n)
     tb = time.perf_counter()
-    idx.build(base, max_batch=args.insert_batch, seed=args.seed,
+    idx.build(base, max_batch=args.insert_batch, seed=args.seed, graph=not args.stream_build,
               progress=lambda d, t: log(f"  GPU build {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
     idx.sync()
     gpu_build_s = time.perf_counter() - tb
@@ -842,8 +842,9 @@ def main():
     ap.add_argument("--build", choices=["cpu", "gpu"], default=None,
                     help="graph from the reference's CPU build or the GPU batched build (default per workload)")
     ap.add_argument("--insert-batch", type=int, default=16384)
-    ap.add_argument("--build-graph", action="store_true",
-                    help="GPU builds run each insert batch's kernel phases as HIP graphs (PK_BUILD_GRAPH)")
+    ap.add_argument("--stream-build", action="store_true",
+                    help="GPU builds launch each insert batch's kernels directly instead of replaying them as HIP "
+                         "graphs (PK_BUILD_GRAPH, the default)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", default=None,
                     help="comma list of low-precision traversal copies (f16, i8) searched + FP32 re-ranked "
@@ -988,11 +989,11 @@ def main():
         idx.set_row_limit(row_limit)
         tb = time.perf_counter()
         idx.build(base, max_batch=args.insert_batch, seed=args.seed, exact_rows=not args.capped_rows,
-                  graph=args.build_graph,
+                  graph=not args.stream_build,
                   progress=lambda d, t: log(f"  inserted {d:,} / {t:,} ({time.perf_counter() - tb:.1f}s)"))
         idx.sync()
         build_info = {"build_seconds": time.perf_counter() - tb, "threads": 0, "where": "gpu",
-                      "hip_graphs": bool(args.build_graph),
+                      "hip_graphs": not args.stream_build,
                       "insert_batch": args.insert_batch, "capped_rows": bool(args.capped_rows), "efc": efc,
                       "row_limit": row_limit or 511}
         log(f"GPU build in {build_info['build_seconds']:.1f}s")
@@ -1275,7 +1276,8 @@ def main():
            "recall_queries": gt_nq, "graph": "reference CPU build" if build == "cpu" else
            f"GPU batched build ({build_info['build_seconds']:.1f}s, efC {build_info.get('efc', efc)}, "
            + ("rows capped at D" if build_info.get('capped_rows') else
-              f"reference rows up to {build_info.get('row_limit', 511)} ids") + ")",
+              f"reference rows up to {build_info.get('row_limit', 511)} ids") +
+           (", insert batches replayed as HIP graphs" if build_info.get("hip_graphs") else "") + ")",
            "search": {"exact": "exact (reference heap order)",
                       "fast": "fast mode (sorted register beam, width-w expansion, LDS visited filter)"}[main_leg],
            "p50_query_latency_ms": round(float(np.percentile(lat_ms, 50)), 4),

@@ -102,3 +102,43 @@ def test_graphs_smaller_than_k(kind, n):
     np.testing.assert_array_equal(loaded.search(queries, k, 32)[0][:, :n], oi[:, :n])
     loaded.close()
     built.close()
+
+
+@pytest.mark.parametrize("kind,metric,dim",
+                         [(k, m, d) for k in ("f32", "f16") for m in (O.L2, O.IP) for d in FLOAT_DIMS] +
+                         [(k, m, d) for k in ("u8", "i8") for m in (O.L2, O.IP) for d in INT_DIMS])
+def test_fast_mode_ragged_dims(kind, metric, dim):
+    """The fast mode on the same graphs: with ef >= N every width returns the
+    exact k smallest distances; on float data (no ties) width 1 returns the
+    exact search's ids and distances at ef 40."""
+    n, nq, M, efc = 400, 40, 8, 40
+    say("fast ragged", kind, "L2" if metric == O.L2 else "IP", dim)
+    base, queries, g = oracle_case(kind, metric, n, nq, dim, M, efc, seed=dim)
+    idx = index_from_oracle(g)
+    for width in (1, 3):
+        idx.set_fast_tuning(width=width)
+        ids, ds, cnt = idx.search_fast(queries, 10, 400)
+        assert (cnt == 10).all()
+        check_exhaustive(ds, base, queries, metric, 10, n)
+        for row in ids:
+            assert len(set(row.tolist())) == 10
+    if kind in ("f32", "f16"):
+        idx.set_fast_tuning(width=1)
+        want = idx.search(queries, 10, 40)
+        got = idx.search_fast(queries, 10, 40)
+        np.testing.assert_array_equal(got[2], want[2])
+        np.testing.assert_array_equal(got[0], want[0])
+        np.testing.assert_array_equal(got[1], want[1])
+    idx.close()
+
+
+@pytest.mark.parametrize("n", [1, 5])
+def test_fast_mode_graphs_smaller_than_k(n):
+    dim, M, efc, k = 24, 4, 16, 10
+    base, queries, g = oracle_case("f32", O.L2, n, 8, dim, M, efc, seed=n)
+    idx = index_from_oracle(g)
+    ids, ds, cnt = idx.search_fast(queries, k, 32)
+    assert (cnt == n).all()
+    assert (ids[:, n:] == -1).all() and np.isinf(ds[:, n:]).all()
+    check_exhaustive(ds, base, queries, O.L2, k, n)
+    idx.close()

@@ -109,8 +109,8 @@ def test_graphs_smaller_than_k(kind, n):
                          [(k, m, d) for k in ("u8", "i8") for m in (O.L2, O.IP) for d in INT_DIMS])
 def test_fast_mode_ragged_dims(kind, metric, dim):
     """The fast mode on the same graphs: with ef >= N every width returns the
-    exact k smallest distances; on float data (no ties) width 1 returns the
-    exact search's ids and distances at ef 40."""
+    exact k smallest distances; on float data, for every query without tied
+    distances, width 1 returns the exact search's ids and distances at ef 40."""
     n, nq, M, efc = 400, 40, 8, 40
     say("fast ragged", kind, "L2" if metric == O.L2 else "IP", dim)
     base, queries, g = oracle_case(kind, metric, n, nq, dim, M, efc, seed=dim)
@@ -123,12 +123,20 @@ def test_fast_mode_ragged_dims(kind, metric, dim):
         for row in ids:
             assert len(set(row.tolist())) == 10
     if kind in ("f32", "f16"):
+        # width 1 equals the exact search for every query whose distances to
+        # all rows differ in float32 (ties are ordered by id in the fast mode,
+        # by heap layout in the reference; f16 at tiny dims ties often)
+        d32 = np.array([[O.distance(q, b, metric) for b in base] for q in queries], np.float32)
+        tie_free = np.array([len(np.unique(r)) == n for r in d32])
+        if dim >= 17:
+            assert tie_free.mean() > 0.9
         idx.set_fast_tuning(width=1)
         want = idx.search(queries, 10, 40)
         got = idx.search_fast(queries, 10, 40)
-        np.testing.assert_array_equal(got[2], want[2])
-        np.testing.assert_array_equal(got[0], want[0])
-        np.testing.assert_array_equal(got[1], want[1])
+        say("  tie-free queries compared:", int(tie_free.sum()), "of", nq)
+        np.testing.assert_array_equal(got[2][tie_free], want[2][tie_free])
+        np.testing.assert_array_equal(got[0][tie_free], want[0][tie_free])
+        np.testing.assert_array_equal(got[1][tie_free], want[1][tie_free])
     idx.close()
 
 

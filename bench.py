@@ -657,13 +657,37 @@ def run_uneven(args):
     graph.close()
     qps = nq * args.steps / wall
     ids = d_ids.to_array(np.int32, (nq, 100))
-    rec = {}
-    for kk in (1, 10, 100):
-        sel = np.where(ks[:ns] == kk)[0]
-        hit = sum(len(set(ids[i, :kk].tolist()) & set(gt[i, :kk].tolist())) for i in sel)
-        rec[f"k={kk}"] = round(hit / (kk * max(1, len(sel))), 4)
+
+    def recall_by_k(ids):
+        rec = {}
+        for kk in (1, 10, 100):
+            sel = np.where(ks[:ns] == kk)[0]
+            hit = sum(len(set(ids[i, :kk].tolist()) & set(gt[i, :kk].tolist())) for i in sel)
+            rec[f"k={kk}"] = round(hit / (kk * max(1, len(sel))), 4)
+        return rec
+
+    rec = recall_by_k(ids)
     log(f"uneven launch: {nq} queries, {args.steps} graph replays: {qps:,.0f} QPS, kernel "
         f"{ev_ms / args.steps:.3f} ms; recall by k {rec}")
+
+    # the same launch in the fast mode (per-query k and ef, ef_max 512)
+    fast = None
+    if int(efs.max()) <= 512:
+        if not args.no_autotune:
+            idx.autotune_fast(d_q, min(nq, 4096), 10, int(np.median(efs)))
+        for _ in range(args.warmup):
+            idx.search_fast_device(d_q, nq, 0, 0, 100, d_ids, d_d, d_c, None, d_k, d_e, int(efs.max()))
+        idx.sync()
+        tf0 = time.perf_counter()
+        for _ in range(args.steps):
+            idx.search_fast_device(d_q, nq, 0, 0, 100, d_ids, d_d, d_c, None, d_k, d_e, int(efs.max()))
+        idx.sync()
+        fwall = time.perf_counter() - tf0
+        if idx.device_error():
+            sys.exit("fast uneven launch: device reported a failure")
+        frec = recall_by_k(d_ids.to_array(np.int32, (nq, 100)))
+        fast = {"qps": round(nq * args.steps / fwall, 1), "recall_by_k": frec}
+        log(f"fast mode, same launch: {fast['qps']:,.0f} QPS; recall by k {frec}")
     line = {
         "metric": "QPS of one uneven launch (per-query ef log-uniform [16, 512], k in {1, 10, 100}), HIP graph",
         "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -671,7 +695,8 @@ def run_uneven(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": wl["desc"], "n": n, "nq": nq, "dim": dim, "M": M, "efC": efc,
                    "ef": "log-uniform [16, 512] per query", "k": "{1, 10, 100} per query",
-                   "recall_by_k": rec, "recall_queries": ns, "gpu_build_seconds": round(gpu_build_s, 2),
+                   "recall_by_k": rec, "fast_mode": fast, "recall_queries": ns,
+                   "gpu_build_seconds": round(gpu_build_s, 2),
                    "reference_cpu_build_seconds": round(cpu_info["build_seconds"], 2), "reference_build_threads":
                    threads, "recall10_vs_ef_gpu_built_vs_reference_built": curve, "autotuned": tuned,
                    "kernel_ms": round(ev_ms / args.steps, 4)},

@@ -1188,8 +1188,7 @@ def main():
     results = {}
     # the exact search: the reference's heap order, the CPU baseline's ef
     ef_x, r_x, sw_x = sweep("exact", args.recall_target, efs)
-    if mode == "exact":
-        autotune("exact", ef_x)
+    autotune("exact", ef_x)  # tuning cannot change the exact search's results
     steps_x = args.steps if mode == "exact" else max(3, args.steps // 4)
     wall_x, ks_x = timed("exact", ef_x, steps_x)
     results["exact"] = {"ef": ef_x, "recall@10": r_x, "qps": round(nq * steps_x / wall_x, 1),

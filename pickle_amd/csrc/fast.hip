@@ -680,6 +680,10 @@ int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq,
     // the filter takes 4 << hb bytes: 2^(hb + 1) entries of 16 bits, each
     // keeping b - (hb + 1) <= 15 bits of the hashed id (more entries when the
     // capacity needs it); with no more ids than entries every id has its own
+    // the hash is a bijection of the ids below 2^b only: indexes past 2^27
+    // ids are refused, as the exact search refuses them
+    if (h->capacity > ((int64_t)1 << 27))
+        return set_error(PK_ERR_CAPACITY, "fast search supports at most 2^27 ids per index");
     int b = 1;
     while (b < 27 && ((int64_t)1 << b) < h->capacity) b++;
     int sb = std::max(hb + 1, b - 15);

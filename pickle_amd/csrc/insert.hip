@@ -857,7 +857,11 @@ static int run_insert_phase(pk_index* h, const InsertArgs& a, int phase, int gri
         }
     }
     hipError_t e = hipSuccess;
-    if (!ex) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (!ex) {
+        hipGraphExec_t fresh = nullptr;
+        e = hipGraphInstantiate(&fresh, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) ex = fresh;
+    }
     if (e == hipSuccess) e = hipGraphLaunch(ex, h->stream);
     (void)hipGraphDestroy(g);  // the executable graph holds its own copy
     if (e != hipSuccess) {

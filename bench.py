@@ -1288,8 +1288,9 @@ def main():
                          f"{'reference-built' if build == 'cpu' else 'GPU-built'} graph",
                "qps_per_thread": round(st["qps"] / threads, 1), "recall@10": round(cpu_recall, 4),
                "ids_identical_to_gpu_exact": round(same, 4), "host": hostcpu,
-               "build_seconds": round(build_info["build_seconds"], 2),
-               "build_threads": build_info["threads"], "build_where": build_info.get("where", "cpu")}
+               "graph_build_seconds": round(build_info["build_seconds"], 2),
+               "graph_built_by": "the reference on the host cores (build_threads)" if build_info.get("where", "cpu")
+               == "cpu" else "the GPU", "build_threads": build_info["threads"]}
         log(f"CPU reference: {st['qps']:,.0f} QPS on {threads} threads at ef={ef_x}, recall {cpu_recall:.4f}, "
             f"GPU exact ids identical on {same * 100:.2f}% of queries; GPU {main_leg} = "
             f"{qps / st['qps']:.1f}x")

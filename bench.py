@@ -1241,7 +1241,15 @@ def main():
     log(f"timed: {args.steps} x {nq} queries at ef={ef_m} ({main_leg}): {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
     # the main leg's results again (the timed launches' outputs)
-    r_ids = d_ids.to_array(np.int32, (nq, k))[:gt_nq]
+    all_ids = d_ids.to_array(np.int32, (nq, k))
+    r_ids = all_ids[:gt_nq]
+    if main_leg != "exact":
+        # the fast leg against the exact search (the reference's results)
+        same = float(np.mean(np.all(all_ids == exact_ids, axis=1)))
+        results[main_leg]["ids_identical_to_exact"] = round(same, 4)
+        results[main_leg]["recall_minus_exact"] = round(results[main_leg]["recall@10"] - r_x, 4)
+        log(f"{main_leg} leg: ids identical to the exact search on {same * 100:.2f}% of queries, recall "
+            f"{results[main_leg]['recall@10']:.4f} vs {r_x:.4f}")
     r_d = d_dist.to_array(np.float32, (nq, k))[:gt_nq].astype(np.float64)
     ok = r_ids >= 0
     rows = base[np.where(ok, r_ids, 0)].astype(np.float64)

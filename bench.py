@@ -1223,6 +1223,9 @@ def main():
     ctr_m = counters(main_leg, ef_m)
     wall, kernel_s = timed(main_leg, ef_m, args.steps)
     grid, lds = handle(main_leg).last_launch()  # the timed kernel's launch shape
+    # the timed launches' outputs, before any other launch reuses the buffers
+    all_ids = d_ids.to_array(np.int32, (nq, k))
+    all_d = d_dist.to_array(np.float32, (nq, k))
     qps = nq * args.steps / wall
     results[main_leg]["qps"] = round(qps, 1)
     results[main_leg]["kernel_ms"] = round(kernel_s * 1e3, 4)
@@ -1240,8 +1243,6 @@ def main():
     lat_ms = ctr_m["latency_ticks"].astype(np.float64) / khz
     log(f"timed: {args.steps} x {nq} queries at ef={ef_m} ({main_leg}): {qps:,.0f} QPS, kernel {kernel_s * 1e3:.3f} ms, "
         f"{achieved:.1f} GB/s algorithmic (grid {grid}, LDS {lds} B/workgroup)")
-    # the main leg's results again (the timed launches' outputs)
-    all_ids = d_ids.to_array(np.int32, (nq, k))
     r_ids = all_ids[:gt_nq]
     if main_leg != "exact":
         # the fast leg against the exact search (the reference's results)
@@ -1250,7 +1251,7 @@ def main():
         results[main_leg]["recall_minus_exact"] = round(results[main_leg]["recall@10"] - r_x, 4)
         log(f"{main_leg} leg: ids identical to the exact search on {same * 100:.2f}% of queries, recall "
             f"{results[main_leg]['recall@10']:.4f} vs {r_x:.4f}")
-    r_d = d_dist.to_array(np.float32, (nq, k))[:gt_nq].astype(np.float64)
+    r_d = all_d[:gt_nq].astype(np.float64)
     ok = r_ids >= 0
     rows = base[np.where(ok, r_ids, 0)].astype(np.float64)
     qv = queries[:gt_nq].astype(np.float64)[:, None, :]

@@ -1145,8 +1145,11 @@ def main():
             t = idx.autotune(d_q, ns, k, ef)
         else:
             t = idx.autotune_fast(d_q, ns, k, min(ef, 512))
+            # the precision forms take the width, filter and workgroup shape;
+            # a cap on resident workgroups found for FP32 rows would starve
+            # their narrower, more latency-bound gathers
             for h in tidx.values():
-                h.set_fast_tuning(t["width"], t["hash_bits"], t["waves_per_group"], t["blocks_per_cu"])
+                h.set_fast_tuning(t["width"], t["hash_bits"], t["waves_per_group"], 0)
         tuned[leg] = t
         log(f"[{leg}] autotuned at ef={ef}: {t}")
 

@@ -85,3 +85,27 @@ def test_brute_force_many_split_lists():
     check(np.uint8, abi.L2, 50_000, 300, 128, 31)
     check(np.float32, abi.L2, 20_000, 130, 128, 32)
     check(np.float16, abi.IP, 20_000, 130, 96, 33)
+
+
+def test_brute_force_many_ties_across_splits():
+    """600 copies of one row, spread over a 150,000-row base, and queries
+    equal to it: every copy is at distance 0 (L2).  The base is split 32
+    ways, so all 64 lists of a query hold some of the copies, and the merge
+    must return the 32 lowest-numbered ones."""
+    rng = np.random.default_rng(44)
+    n, dim = 150_000, 128
+    base = rng.integers(0, 256, (n, dim)).astype(np.uint8)
+    where = np.sort(rng.choice(n, 600, replace=False))
+    base[where] = base[where[0]]
+    q = np.repeat(base[where[:1]], 70, axis=0)
+    q[35:] = rng.integers(0, 256, (35, dim)).astype(np.uint8)
+    say("brute force many ties: 600 copies, 70 queries")
+    idx = abi.Index(np.uint8, abi.L2, dim, 8, 10, n)
+    idx.set_base(base)
+    ids, ds = idx.brute_force(q)
+    idx.close()
+    np.testing.assert_array_equal(ids[:35], np.broadcast_to(where[:K], (35, K)))
+    assert (ds[:35] == 0).all()
+    gt_ids, gt_d = O.brute_force(base, q, K, O.L2)
+    np.testing.assert_array_equal(ids, gt_ids)
+    np.testing.assert_array_equal(ds, gt_d.astype(np.float32))

@@ -260,7 +260,8 @@ void complete_slot(pk_server* S, Slot& s, int status) {
             if (!bad.empty()) {
                 rs = PK_ERR_DEVICE;
                 err = std::to_string(bad.size()) +
-                      " queries failed on the device (a visited table of 2^20 slots filled up)";
+                      (S->fast ? " queries failed on the device (fast mode)"
+                               : " queries failed on the device (a visited table of 2^20 slots filled up)");
             }
         }
         if (r->parent)

@@ -74,14 +74,32 @@ __global__ __launch_bounds__(256) void k_brute_rowconst(const uint8_t* __restric
 // ---- integer tiles -------------------------------------------------------
 
 // Queries are staged NKB * 32 bytes apart (u8 padded with 0x80, i8 with 0).
-// Grid: x = query blocks of kIntQ, y = row splits of tiles_per_split tiles.
-// Lists: pd / pi [q][y * 2 + lane / 32][kBruteK].
+// One-dimensional grid of qblocks x splits workgroups: query blocks of kIntQ
+// queries times row splits of tiles_per_split tiles (int_block_coords).
+// Lists: pd / pi [q][split * 2 + lane / 32][kBruteK].
+// Workgroups reach the XCDs in turn (linear id mod kXcds), and each XCD has
+// its own L2.  With a split count that is a multiple of kXcds, workgroup L
+// takes split (L mod kXcds) + kXcds * (L / kXcds / qblocks): every XCD works
+// through whole splits, so all its resident workgroups stream the same rows
+// and share them in its L2.  Otherwise the split is L / qblocks.
+__device__ __forceinline__ void int_block_coords(int64_t qblocks, int xcd_map, int64_t& qb, int64_t& split) {
+    const int64_t L = blockIdx.x;
+    if (xcd_map) {
+        const int64_t slot = L / kXcds;
+        qb = slot % qblocks;
+        split = L % kXcds + kXcds * (slot / qblocks);
+    } else {
+        qb = L % qblocks;
+        split = L / qblocks;
+    }
+}
+
 template <int DT, int MET, int NKB>
 __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
                                                     const int32_t* __restrict__ crow, const uint8_t* __restrict__ qs,
-                                                    int64_t nq, int dim, int64_t tiles_per_split, int nlists,
-                                                    double* __restrict__ pd, int32_t* __restrict__ pi,
-                                                    int32_t* __restrict__ qbound) {
+                                                    int64_t nq, int dim, int64_t qblocks, int xcd_map,
+                                                    int64_t tiles_per_split, int nlists, double* __restrict__ pd,
+                                                    int32_t* __restrict__ pi, int32_t* __restrict__ qbound) {
     constexpr int KB = kBruteK;
     constexpr int QST = NKB * 32;
     constexpr uint32_t FLIP = DT == DT_U8 ? 0x80808080u : 0u;
@@ -89,7 +107,9 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
     constexpr bool CROW = !(DT == DT_I8 && MET == MET_IP);
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5, col = lane & 31;
-    const int64_t q = (int64_t)blockIdx.x * kIntQ + (threadIdx.x >> 6) * 32 + col;
+    int64_t qb, split;
+    int_block_coords(qblocks, xcd_map, qb, split);
+    const int64_t q = qb * kIntQ + (threadIdx.x >> 6) * 32 + col;
     const int64_t qc = q < nq ? q : nq - 1;
 
     v4i bq[NKB];
@@ -123,7 +143,7 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
     int gbound = INT_MAX;  // smallest 32nd-best key any list of this query has published
 
     const int64_t ntiles = (nbase + 31) / 32;
-    const int64_t t0 = (int64_t)blockIdx.y * tiles_per_split;
+    const int64_t t0 = split * tiles_per_split;
     const int64_t t1 = t0 + tiles_per_split < ntiles ? t0 + tiles_per_split : ntiles;
 
     // Base-row tiles arrive two ahead of their use (three register
@@ -218,7 +238,7 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
     }
 
     if (q < nq) {
-        const size_t o = ((size_t)q * nlists + (size_t)blockIdx.y * 2 + h) * KB;
+        const size_t o = ((size_t)q * nlists + (size_t)split * 2 + h) * KB;
 #pragma unroll
         for (int j = 0; j < KB; j++) {
             pd[o + j] = ki[j] == INT_MAX ? __builtin_inf() : (double)(kd[j] + cq);
@@ -273,11 +293,12 @@ template <int DT, int MET>
 static hipError_t launch_int(int nkb, const BrutePlan& p, hipStream_t s, const uint8_t* base, int64_t nbase,
                              int vstride, const int32_t* crow, const uint8_t* dq, int64_t nq, int dim, double* pd,
                              int32_t* pi, int32_t* qbound) {
-    dim3 grid((unsigned)p.qblocks, (unsigned)p.splits);
+    const dim3 grid((unsigned)(p.qblocks * p.splits));
+    const int xcd_map = p.splits % kXcds == 0;
 #define PK_IMMA_CASE(N)                                                                                       \
     if (nkb <= N) {                                                                                           \
         hipLaunchKernelGGL((k_brute_imma<DT, MET, N>), grid, dim3(256), 0, s, base, nbase, vstride, crow, dq, \
-                           nq, dim, p.tiles_per_split, p.nlists, pd, pi, qbound);                            \
+                           nq, dim, p.qblocks, xcd_map, p.tiles_per_split, p.nlists, pd, pi, qbound);        \
         return hipGetLastError();                                                                             \
     }
     PK_IMMA_CASE(1)
@@ -308,7 +329,7 @@ extern "C" int pk_brute_force(pk_index* h, const void* queries, int64_t nq, int3
     static const int kRound[9] = {1, 1, 2, 3, 4, 6, 6, 8, 8};
     const int nu = kRound[nunits];
     const int qst = nu * unit;
-    const BrutePlan p = is_int ? plan_brute(nq, nbase, kIntQ, 32, 2) : plan_brute(nq, nbase, kFltQ, 16, 4);
+    const BrutePlan p = is_int ? plan_brute_int(nq, nbase) : plan_brute(nq, nbase, kFltQ, 16, 4);
     const int64_t npad = (nbase + 31) / 32 * 32;
 
     std::vector<uint8_t> staged((size_t)nq * qst, h->dtype == PK_UINT8 ? 0x80 : 0x00);

@@ -18,6 +18,10 @@ constexpr int kMaxLists = 64;  // lists per query that one merge wave takes
 #ifndef PK_BRUTE_WGS
 #define PK_BRUTE_WGS 512
 #endif
+#ifndef PK_BRUTE_INT_WGS
+#define PK_BRUTE_INT_WGS 1024
+#endif
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -84,6 +88,29 @@ inline BrutePlan plan_brute(int64_t nq, int64_t nbase, int q_per_block, int rows
     p.tiles_per_split = (ntiles + s - 1) / s;
     p.splits = (ntiles + p.tiles_per_split - 1) / p.tiles_per_split;
     p.nlists = (int)(p.splits * lists_per_split);
+    return p;
+}
+
+// Integer tiles: enough splits for PK_BRUTE_INT_WGS workgroups (up to four
+// resident per CU at the kernel's register count), at most kMaxLists / 2.
+// From kXcds / 2 splits up, the count is rounded up to a multiple of kXcds
+// when the rows allow it, so every XCD takes whole splits (brute.hip,
+// int_block_coords); below that, rounding would more than double the
+// lists each query builds.
+inline BrutePlan plan_brute_int(int64_t nq, int64_t nbase) {
+    BrutePlan p;
+    p.qblocks = (nq + kIntQ - 1) / kIntQ;
+    const int64_t ntiles = (nbase + 31) / 32;
+    int64_t s = (PK_BRUTE_INT_WGS + p.qblocks - 1) / p.qblocks;
+    const int64_t smax = kMaxLists / 2;
+    s = s < smax ? s : smax;
+    const int64_t up = (s + kXcds - 1) / kXcds * kXcds;
+    if (s >= kXcds / 2 && up <= smax && up <= ntiles) s = up;
+    s = s < ntiles ? s : ntiles;
+    s = s > 1 ? s : 1;
+    p.tiles_per_split = (ntiles + s - 1) / s;
+    p.splits = (ntiles + p.tiles_per_split - 1) / p.tiles_per_split;
+    p.nlists = (int)(p.splits * 2);
     return p;
 }
 

@@ -71,6 +71,7 @@ def test_brute_force_shapes(dt, metric):
     check(dt, metric, 3000, 70, 100, 11)      # ragged dim, ragged tile, partial query block
     check(dt, metric, 1000, 131, 128, 12)     # exact K blocks
     check(dt, metric, 45, 5, 37, 13)          # one split, odd dim
+    check(dt, metric, 180, 5, 37, 15)         # six splits: fewer tiles than a multiple of the 8 XCDs
     check(dt, metric, 20, 3, 16, 14)          # fewer rows than returned
 
 

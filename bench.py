@@ -279,9 +279,10 @@ def summarize(args):
                  f"adjacency entries scanned, {apq['dists']} distances."]
         if "kernel_per_query" in roof:
             kq = roof["kernel_per_query"]
+            why = ("ids its LDS visited filter forgot are read again" if kern == "k_search_fast"
+                   else "the reference's own counts: its visited set is exact")
             body += [f"The kernel's own counters per query: {kq['hops']} hops, {kq['dists']} distances "
-                     f"(ids its LDS visited filter forgot are read again): "
-                     f"{roof['kernel_counted_bytes_per_launch'] / 1e9:.3f} GB per launch."]
+                     f"({why}): {roof['kernel_counted_bytes_per_launch'] / 1e9:.3f} GB per launch."]
     if args.summarize_pmc:
         fdir, wdir = args.summarize_pmc
         fetch = _counter_per_dispatch(fdir, "FETCH_SIZE", last=bl["steps"], kernel=kern)

@@ -75,25 +75,8 @@ __global__ __launch_bounds__(256) void k_brute_rowconst(const uint8_t* __restric
 
 // Queries are staged NKB * 32 bytes apart (u8 padded with 0x80, i8 with 0).
 // One-dimensional grid of qblocks x splits workgroups: query blocks of kIntQ
-// queries times row splits of tiles_per_split tiles (int_block_coords).
+// queries times row splits of tiles_per_split tiles (brute_block_coords).
 // Lists: pd / pi [q][split * 2 + lane / 32][kBruteK].
-// Workgroups reach the XCDs in turn (linear id mod kXcds), and each XCD has
-// its own L2.  With a split count that is a multiple of kXcds, workgroup L
-// takes split (L mod kXcds) + kXcds * (L / kXcds / qblocks): every XCD works
-// through whole splits, so all its resident workgroups stream the same rows
-// and share them in its L2.  Otherwise the split is L / qblocks.
-__device__ __forceinline__ void int_block_coords(int64_t qblocks, int xcd_map, int64_t& qb, int64_t& split) {
-    const int64_t L = blockIdx.x;
-    if (xcd_map) {
-        const int64_t slot = L / kXcds;
-        qb = slot % qblocks;
-        split = L % kXcds + kXcds * (slot / qblocks);
-    } else {
-        qb = L % qblocks;
-        split = L / qblocks;
-    }
-}
-
 template <int DT, int MET, int NKB>
 __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
                                                     const int32_t* __restrict__ crow, const uint8_t* __restrict__ qs,
@@ -108,7 +91,7 @@ __global__ __launch_bounds__(256) void k_brute_imma(const uint8_t* __restrict__ 
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5, col = lane & 31;
     int64_t qb, split;
-    int_block_coords(qblocks, xcd_map, qb, split);
+    brute_block_coords(qblocks, xcd_map, qb, split);
     const int64_t q = qb * kIntQ + (threadIdx.x >> 6) * 32 + col;
     const int64_t qc = q < nq ? q : nq - 1;
 
@@ -329,7 +312,8 @@ extern "C" int pk_brute_force(pk_index* h, const void* queries, int64_t nq, int3
     static const int kRound[9] = {1, 1, 2, 3, 4, 6, 6, 8, 8};
     const int nu = kRound[nunits];
     const int qst = nu * unit;
-    const BrutePlan p = is_int ? plan_brute_int(nq, nbase) : plan_brute(nq, nbase, kFltQ, 16, 4);
+    const BrutePlan p = is_int ? plan_brute_xcd(nq, nbase, kIntQ, 32, 2, PK_BRUTE_INT_WGS)
+                               : plan_brute_xcd(nq, nbase, kFltQ, 16, 4, PK_BRUTE_WGS);
     const int64_t npad = (nbase + 31) / 32 * 32;
 
     std::vector<uint8_t> staged((size_t)nq * qst, h->dtype == PK_UINT8 ? 0x80 : 0x00);

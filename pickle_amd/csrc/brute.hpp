@@ -15,6 +15,8 @@ constexpr int kBruteK = 32;    // candidates returned per query
 constexpr int kIntQ = 128;     // queries per workgroup, integer tiles (4 waves x 32)
 constexpr int kFltQ = 64;      // queries per workgroup, FP64 tiles (4 waves x 16)
 constexpr int kMaxLists = 64;  // lists per query that one merge wave takes
+// workgroups the grids aim for: FP64 tiles, integer tiles (at their
+// register counts, about two and four resident per CU)
 #ifndef PK_BRUTE_WGS
 #define PK_BRUTE_WGS 512
 #endif
@@ -73,16 +75,21 @@ struct BrutePlan {
     int nlists;
 };
 
-inline BrutePlan plan_brute(int64_t nq, int64_t nbase, int q_per_block, int rows_per_tile, int lists_per_split) {
+// Splits for the XCD-aware grid: enough for target_wgs workgroups, at
+// most kMaxLists / lists_per_split.  From kXcds / 2 splits up, the count is
+// rounded up to a multiple of kXcds when the rows allow it, so every XCD
+// takes whole splits (brute_block_coords); below that, rounding would more
+// than double the lists each query builds.
+inline BrutePlan plan_brute_xcd(int64_t nq, int64_t nbase, int q_per_block, int rows_per_tile, int lists_per_split,
+                                int64_t target_wgs) {
     BrutePlan p;
     p.qblocks = (nq + q_per_block - 1) / q_per_block;
     const int64_t ntiles = (nbase + rows_per_tile - 1) / rows_per_tile;
-    // Every (query, split) pair builds its own lists, and the inserts that
-    // cost grow with the splits, so split only as far as filling the chip
-    // needs: PK_BRUTE_WGS workgroups of 4 waves.
-    int64_t s = (PK_BRUTE_WGS + p.qblocks - 1) / p.qblocks;
+    int64_t s = (target_wgs + p.qblocks - 1) / p.qblocks;
     const int64_t smax = kMaxLists / lists_per_split;
     s = s < smax ? s : smax;
+    const int64_t up = (s + kXcds - 1) / kXcds * kXcds;
+    if (s >= kXcds / 2 && up <= smax && up <= ntiles) s = up;
     s = s < ntiles ? s : ntiles;
     s = s > 1 ? s : 1;
     p.tiles_per_split = (ntiles + s - 1) / s;
@@ -91,30 +98,26 @@ inline BrutePlan plan_brute(int64_t nq, int64_t nbase, int q_per_block, int rows
     return p;
 }
 
-// Integer tiles: enough splits for PK_BRUTE_INT_WGS workgroups (up to four
-// resident per CU at the kernel's register count), at most kMaxLists / 2.
-// From kXcds / 2 splits up, the count is rounded up to a multiple of kXcds
-// when the rows allow it, so every XCD takes whole splits (brute.hip,
-// int_block_coords); below that, rounding would more than double the
-// lists each query builds.
-inline BrutePlan plan_brute_int(int64_t nq, int64_t nbase) {
-    BrutePlan p;
-    p.qblocks = (nq + kIntQ - 1) / kIntQ;
-    const int64_t ntiles = (nbase + 31) / 32;
-    int64_t s = (PK_BRUTE_INT_WGS + p.qblocks - 1) / p.qblocks;
-    const int64_t smax = kMaxLists / 2;
-    s = s < smax ? s : smax;
-    const int64_t up = (s + kXcds - 1) / kXcds * kXcds;
-    if (s >= kXcds / 2 && up <= smax && up <= ntiles) s = up;
-    s = s < ntiles ? s : ntiles;
-    s = s > 1 ? s : 1;
-    p.tiles_per_split = (ntiles + s - 1) / s;
-    p.splits = (ntiles + p.tiles_per_split - 1) / p.tiles_per_split;
-    p.nlists = (int)(p.splits * 2);
-    return p;
+// Workgroups reach the XCDs in turn (linear id mod kXcds), and each XCD has
+// its own L2.  On a one-dimensional grid of qblocks x splits workgroups
+// with a split count that is a multiple of kXcds (xcd_map), workgroup L
+// takes split (L mod kXcds) + kXcds * (L / kXcds / qblocks): every XCD
+// works through whole splits, so all its resident workgroups stream the
+// same rows and share them in its L2.  Otherwise the split is L / qblocks.
+__device__ __forceinline__ void brute_block_coords(int64_t qblocks, int xcd_map, int64_t& qb, int64_t& split) {
+    const int64_t L = blockIdx.x;
+    if (xcd_map) {
+        const int64_t slot = L / kXcds;
+        qb = slot % qblocks;
+        split = L % kXcds + kXcds * (slot / qblocks);
+    } else {
+        qb = L % qblocks;
+        split = L / qblocks;
+    }
 }
 
-// FP64 tiles (brute_fp64.hip): row norms when L2, then the tile kernel
+// FP64 tiles (brute_fp64.hip): row norms when L2, then the tile kernel on
+// the XCD-aware grid of plan p
 hipError_t launch_brute_fp64(int dtype, int metric, int nsc, const BrutePlan& p, hipStream_t s,
                              const uint8_t* base, int64_t nbase, int vstride, int dim, int64_t npad,
                              double* nrm, const uint8_t* dq, int64_t nq, double* pd, int32_t* pi);

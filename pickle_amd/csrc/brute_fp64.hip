@@ -30,18 +30,22 @@ __global__ __launch_bounds__(256) void k_brute_rownorm(const uint8_t* __restrict
 // ---- FP64 tiles ----------------------------------------------------------
 
 // Queries are staged NSC * 64 bytes apart, zero padded.
-// Grid: x = query blocks of kFltQ, y = row splits.  Lists: [q][y * 4 + lane / 16][kBruteK].
+// One-dimensional grid of qblocks x splits workgroups (brute_block_coords):
+// query blocks of kFltQ queries times row splits of tiles_per_split tiles.
+// Lists: [q][split * 4 + lane / 16][kBruteK].
 template <int DT, int MET, int NSC>
 __global__ __launch_bounds__(256) void k_brute_dmma(const uint8_t* __restrict__ base, int64_t nbase, int vstride,
                                                     const double* __restrict__ nrm, const uint8_t* __restrict__ qs,
-                                                    int64_t nq, int64_t tiles_per_split, int nlists,
-                                                    double* __restrict__ pd, int32_t* __restrict__ pi) {
+                                                    int64_t nq, int64_t qblocks, int xcd_map, int64_t tiles_per_split,
+                                                    int nlists, double* __restrict__ pd, int32_t* __restrict__ pi) {
     constexpr int KB = kBruteK;
     constexpr int QST = NSC * 64;
     constexpr int E = DT == DT_F32 ? 4 : 8;
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, col = lane & 15;
-    const int64_t q = (int64_t)blockIdx.x * kFltQ + (threadIdx.x >> 6) * 16 + col;
+    int64_t qb, split;
+    brute_block_coords(qblocks, xcd_map, qb, split);
+    const int64_t q = qb * kFltQ + (threadIdx.x >> 6) * 16 + col;
     const int64_t qc = q < nq ? q : nq - 1;
 
     uint4 bq[NSC];
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(256) void k_brute_dmma(const uint8_t* __restrict__ 
     double thr = __builtin_inf();
 
     const int64_t ntiles = (nbase + 15) / 16;
-    const int64_t t0 = (int64_t)blockIdx.y * tiles_per_split;
+    const int64_t t0 = split * tiles_per_split;
     const int64_t t1 = t0 + tiles_per_split < ntiles ? t0 + tiles_per_split : ntiles;
 
     uint4 an[NSC];
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(256) void k_brute_dmma(const uint8_t* __restrict__ 
     }
 
     if (q < nq) {
-        const size_t o = ((size_t)q * nlists + (size_t)blockIdx.y * 4 + g) * KB;
+        const size_t o = ((size_t)q * nlists + (size_t)split * 4 + g) * KB;
 #pragma unroll
         for (int j = 0; j < KB; j++) {
             pd[o + j] = ki[j] == INT_MAX ? __builtin_inf() : (MET == MET_L2 ? kd[j] + qq : kd[j]);
@@ -141,11 +145,12 @@ __global__ __launch_bounds__(256) void k_brute_dmma(const uint8_t* __restrict__ 
 template <int DT, int MET>
 static hipError_t launch_flt(int nsc, const BrutePlan& p, hipStream_t s, const uint8_t* base, int64_t nbase,
                              int vstride, const double* nrm, const uint8_t* dq, int64_t nq, double* pd, int32_t* pi) {
-    dim3 grid((unsigned)p.qblocks, (unsigned)p.splits);
+    const dim3 grid((unsigned)(p.qblocks * p.splits));
+    const int xcd_map = p.splits % kXcds == 0;
 #define PK_DMMA_CASE(N)                                                                                         \
     if (nsc <= N) {                                                                                             \
         hipLaunchKernelGGL((k_brute_dmma<DT, MET, N>), grid, dim3(256), 0, s, base, nbase, vstride, nrm, dq, nq, \
-                           p.tiles_per_split, p.nlists, pd, pi);                                               \
+                           p.qblocks, xcd_map, p.tiles_per_split, p.nlists, pd, pi);                           \
         return hipGetLastError();                                                                               \
     }
     PK_DMMA_CASE(1)

@@ -46,6 +46,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pickle_gpu.h"
@@ -83,6 +84,7 @@ struct FastArgs {
     int32_t max_rounds;  // per query; past it the query fails (flag 1)
     int32_t* qcounter;
     int32_t* err;
+    const int32_t* order;  // queue position -> query (null: in order)
 };
 
 // ---- keys: order-preserving distance bits, then id, then the pending bit
@@ -454,8 +456,9 @@ __global__ __launch_bounds__(256, 4) void k_search_fast(FastArgs A) {
     for (;;) {
         int qi = 0;
         if (leader) qi = atomicAdd(A.qcounter, 1);
-        const int64_t q = readlane(qi, 0);
-        if (q >= A.nq) break;
+        const int64_t pos = readlane(qi, 0);
+        if (pos >= A.nq) break;
+        const int64_t q = A.order ? (int64_t)uni(gbl(A.order)[pos]) : pos;
         const uint64_t t_begin = wall_clock64();
         stage_row(qst, A.queries + (size_t)q * (size_t)A.qbytes, A.qbytes, A.vstride, lane);
         DF df;
@@ -595,21 +598,43 @@ static hipError_t launch_rr(const FastArgs& a, int rr, int grid, int block, size
     return hipGetLastError();
 }
 
-template <int DT, int MET>
-static hipError_t launch_fast_dt(const FastArgs& a, int size_class, int rr, int grid, int block, size_t lds,
-                                 hipStream_t s) {
+template <class T>
+struct TypeTag {
+    using type = T;
+};
+
+// Calls f(TypeTag<DF>, std::bool_constant<INT>) with the distance functor
+// the fast mode uses for this element type, metric and row size class.
+template <int DT, int MET, class F>
+static hipError_t with_fast_df(int size_class, F&& f) {
     constexpr bool INT = DT == DT_U8 || DT == DT_I8;
     if constexpr (INT) {
-        if (size_class == 0) return launch_rr<IntDist<DT, MET, 1>, true>(a, rr, grid, block, lds, s);
-        if (size_class == 1) return launch_rr<IntDist<DT, MET, 2>, true>(a, rr, grid, block, lds, s);
-        return launch_rr<IntDist<DT, MET, 4>, true>(a, rr, grid, block, lds, s);
+        if (size_class == 0) return f(TypeTag<IntDist<DT, MET, 1>>{}, std::true_type{});
+        if (size_class == 1) return f(TypeTag<IntDist<DT, MET, 2>>{}, std::true_type{});
+        return f(TypeTag<IntDist<DT, MET, 4>>{}, std::true_type{});
     } else {
         if constexpr (DT == DT_F16) {
-            if (size_class == kHalfFastClass) return launch_rr<HalfDist<MET, 1>, true>(a, rr, grid, block, lds, s);
-            if (size_class == kHalfFastClass + 1) return launch_rr<HalfDist<MET, 2>, true>(a, rr, grid, block, lds, s);
+            if (size_class == kHalfFastClass) return f(TypeTag<HalfDist<MET, 1>>{}, std::true_type{});
+            if (size_class == kHalfFastClass + 1) return f(TypeTag<HalfDist<MET, 2>>{}, std::true_type{});
         }
-        if (size_class == 0) return launch_rr<FloatDist<DT, MET, 8>, false>(a, rr, grid, block, lds, s);
-        return launch_rr<FloatDist<DT, MET, 16>, false>(a, rr, grid, block, lds, s);
+        if (size_class == 0) return f(TypeTag<FloatDist<DT, MET, 8>>{}, std::false_type{});
+        return f(TypeTag<FloatDist<DT, MET, 16>>{}, std::false_type{});
+    }
+}
+
+// with_fast_df for a run-time element type and metric
+template <class F>
+static hipError_t with_fast_df(int dtype, int metric, int size_class, F&& f) {
+    switch (dtype * 4 + metric) {
+        case DT_U8 * 4 + MET_L2: return with_fast_df<DT_U8, MET_L2>(size_class, f);
+        case DT_U8 * 4 + MET_IP: return with_fast_df<DT_U8, MET_IP>(size_class, f);
+        case DT_I8 * 4 + MET_L2: return with_fast_df<DT_I8, MET_L2>(size_class, f);
+        case DT_I8 * 4 + MET_IP: return with_fast_df<DT_I8, MET_IP>(size_class, f);
+        case DT_F32 * 4 + MET_L2: return with_fast_df<DT_F32, MET_L2>(size_class, f);
+        case DT_F32 * 4 + MET_IP: return with_fast_df<DT_F32, MET_IP>(size_class, f);
+        case DT_F16 * 4 + MET_L2: return with_fast_df<DT_F16, MET_L2>(size_class, f);
+        case DT_F16 * 4 + MET_IP: return with_fast_df<DT_F16, MET_IP>(size_class, f);
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -619,17 +644,100 @@ static hipError_t launch_fast(const FastArgs& a, int dtype, int metric, int size
     hipLaunchKernelGGL(k_fill_i32_fast, dim3(1), dim3(64), 0, s, a.err, 0, 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    switch (dtype * 4 + metric) {
-        case DT_U8 * 4 + MET_L2: return launch_fast_dt<DT_U8, MET_L2>(a, size_class, rr, grid, block, lds, s);
-        case DT_U8 * 4 + MET_IP: return launch_fast_dt<DT_U8, MET_IP>(a, size_class, rr, grid, block, lds, s);
-        case DT_I8 * 4 + MET_L2: return launch_fast_dt<DT_I8, MET_L2>(a, size_class, rr, grid, block, lds, s);
-        case DT_I8 * 4 + MET_IP: return launch_fast_dt<DT_I8, MET_IP>(a, size_class, rr, grid, block, lds, s);
-        case DT_F32 * 4 + MET_L2: return launch_fast_dt<DT_F32, MET_L2>(a, size_class, rr, grid, block, lds, s);
-        case DT_F32 * 4 + MET_IP: return launch_fast_dt<DT_F32, MET_IP>(a, size_class, rr, grid, block, lds, s);
-        case DT_F16 * 4 + MET_L2: return launch_fast_dt<DT_F16, MET_L2>(a, size_class, rr, grid, block, lds, s);
-        case DT_F16 * 4 + MET_IP: return launch_fast_dt<DT_F16, MET_IP>(a, size_class, rr, grid, block, lds, s);
-        default: return hipErrorInvalidValue;
+    return with_fast_df(dtype, metric, size_class, [&](auto tag, auto in) {
+        return launch_rr<typename decltype(tag)::type, decltype(in)::value>(a, rr, grid, block, lds, s);
+    });
+}
+
+// ---- query order -----------------------------------------------------------
+//
+// Queries of one region of the data visit overlapping rows, and rows read by
+// one query are still in the L2 or the MI355X's 256 MB Infinity Cache when
+// another query of the region asks for them.  For a large batch the
+// queries are therefore handed out region by region: each query descends
+// the upper layers as the search does (greedy_fast) down to a key layer
+// whose node count gives at least kOrderQueriesPerKey queries per node on
+// average, the node it stops at is its key, and a counting sort of the keys
+// gives the order.  Results do not depend on the order.
+constexpr int64_t kOrderMinQueries = 8192;
+constexpr int64_t kOrderQueriesPerKey = 64;
+
+struct OrderArgs {
+    const LayerDev* layers;
+    int32_t enter_level;
+    int32_t key_level;
+    int32_t nkeys;
+    int32_t dim;
+    int32_t vstride;
+    int32_t qbytes;
+    int32_t lds_per_wave;
+    const uint8_t* queries;
+    int64_t nq;
+    int32_t* keys;
+    int32_t* ranks;
+    int32_t* hist;
+};
+
+template <class DF, bool IS_INT>
+__global__ __launch_bounds__(256) void k_order_keys(OrderArgs A) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = lane_id();
+    const int w = threadIdx.x >> 6;
+    uint8_t* my = smem + (size_t)w * A.lds_per_wave;
+    int* cl = reinterpret_cast<int*>(my);
+    float* cd = reinterpret_cast<float*>(cl + kWave);
+    uint8_t* qst = reinterpret_cast<uint8_t*>(cd + kWave);
+    const int g = lane % DF::LPC;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + w; q < A.nq; q += (int64_t)gridDim.x * 4) {
+        stage_row(qst, A.queries + (size_t)q * (size_t)A.qbytes, A.qbytes, A.vstride, lane);
+        DF df;
+        df.load_query(qst, g, IS_INT ? (A.vstride >> 4) : A.dim);
+        WorkCount wc;
+        int cur = 0;
+        for (int level = A.enter_level; level > A.key_level; level--) {
+            const LayerDev L = A.layers[level];
+            cur = greedy_fast(df, L, A.vstride, cur, cl, cd, lane, wc);
+            cur = gbl(L.down)[cur];
+        }
+        cur = greedy_fast(df, A.layers[A.key_level], A.vstride, cur, cl, cd, lane, wc);
+        const int key = cur < A.nkeys ? cur : A.nkeys - 1;
+        if (lane == 0) {
+            A.keys[q] = key;
+            A.ranks[q] = atomicAdd(&A.hist[key], 1);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+}
+
+// hist[0, n) -> exclusive prefix sums, in one workgroup
+__global__ __launch_bounds__(1024) void k_order_scan(int32_t* hist, int64_t n) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b = t * per < n ? t * per : n;
+    const int64_t e = b + per < n ? b + per : n;
+    int64_t sum = 0;
+    for (int64_t i = b; i < e; i++) sum += hist[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;
+    for (int64_t i = b; i < e; i++) {
+        const int32_t c = hist[i];
+        hist[i] = (int32_t)run;
+        run += c;
+    }
+}
+
+__global__ void k_order_scatter(const int32_t* __restrict__ keys, const int32_t* __restrict__ ranks,
+                                const int32_t* __restrict__ offsets, int32_t* __restrict__ order, int64_t nq) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < nq) order[offsets[keys[q]] + ranks[q]] = (int32_t)q;
 }
 
 // rows of the beam for an ef: 64, 128, 256 or 512 keys
@@ -657,6 +765,85 @@ using namespace pk;
 
 namespace pk {
 int search_class_of(const pk_index* h);
+
+// The order for a batch of nq queries (order_queries in the comment above
+// kOrderMinQueries), or null when the batch is too small, no upper layer
+// gives enough queries per key, or the stream is being captured (the
+// buffers may need to grow).
+static int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, int size_class,
+                         const int32_t** out) {
+    *out = nullptr;
+    if (nq < kOrderMinQueries) return PK_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c.stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return PK_OK;
+    const int enter = h->enter_level;
+    int lk = 0;
+    for (int l = 1; l <= enter && l < PK_MAX_LEVELS; l++) {
+        const int64_t n = h->layers[l].n;
+        if (n >= 2 && n <= nq / kOrderQueriesPerKey) {
+            lk = l;
+            break;
+        }
+    }
+    if (lk == 0) return PK_OK;
+    const int64_t nkeys = h->layers[lk].n;
+    const size_t need = (size_t)3 * (size_t)nq + (size_t)nkeys;
+    if (c.order_ints < need) {
+        if (c.order) {
+            // an earlier launch on this stream may still read the old buffer
+            if (hipStreamSynchronize(c.stream) != hipSuccess) return set_error(PK_ERR_HIP, "query order: stream sync");
+            (void)hipFree(c.order);
+            c.order = nullptr;
+            c.order_ints = 0;
+        }
+        if (hipMalloc(reinterpret_cast<void**>(&c.order), need * sizeof(int32_t)) != hipSuccess) {
+            c.order = nullptr;
+            return set_error(PK_ERR_NOMEM, "query order buffers");
+        }
+        c.order_ints = need;
+    }
+    int32_t* keys = c.order;
+    int32_t* ranks = keys + nq;
+    int32_t* order = ranks + nq;
+    int32_t* hist = order + nq;
+    OrderArgs a{};
+    a.layers = static_cast<const LayerDev*>(h->d_layers);
+    a.enter_level = enter;
+    a.key_level = lk;
+    a.nkeys = (int32_t)nkeys;
+    a.dim = h->dim;
+    a.vstride = h->vstride;
+    a.qbytes = h->qbytes;
+    const size_t per_wave = (size_t)kWave * 8 + ((size_t)h->vstride + 15) / 16 * 16;
+    a.lds_per_wave = (int32_t)per_wave;
+    a.queries = static_cast<const uint8_t*>(d_queries);
+    a.nq = nq;
+    a.keys = keys;
+    a.ranks = ranks;
+    a.hist = hist;
+    const int grid = (int)std::min<int64_t>((nq + 3) / 4, (int64_t)h->num_cus * 8);
+    hipLaunchKernelGGL(k_fill_i32_fast, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, c.stream, hist, 0,
+                       (int)nkeys);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = with_fast_df(h->dtype, h->metric, size_class, [&](auto tag, auto in) {
+            hipLaunchKernelGGL((k_order_keys<typename decltype(tag)::type, decltype(in)::value>), dim3(grid),
+                               dim3(256), per_wave * 4, c.stream, a);
+            return hipGetLastError();
+        });
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, c.stream, hist, nkeys);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_order_scatter, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c.stream, keys, ranks,
+                           hist, order, nq);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("query order launch: ") + hipGetErrorString(e));
+    *out = order;
+    return PK_OK;
+}
 
 int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, const int32_t* d_k,
                    const int32_t* d_ef, int32_t k_all, int32_t ef_all, int32_t ef_max, int32_t k_stride,
@@ -729,8 +916,9 @@ int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq,
     a.max_rounds = (int)std::min<int64_t>(1 << 30, 4 * h->capacity + 64);
     a.qcounter = c.qcounter;
     a.err = c.err;
-    const hipError_t e =
-        launch_fast(a, h->dtype, h->metric, search_class_of(h), rr, grid, 64 * wpg, lds, c.stream);
+    const int size_class = search_class_of(h);
+    if ((rc = order_queries(h, c, d_queries, nq, size_class, &a.order))) return rc;
+    const hipError_t e = launch_fast(a, h->dtype, h->metric, size_class, rr, grid, 64 * wpg, lds, c.stream);
     if (e != hipSuccess) return set_error(PK_ERR_HIP, std::string("fast search launch: ") + hipGetErrorString(e));
     if (&c == &h->sc) {
         h->last_grid = grid;

@@ -485,7 +485,8 @@ void free_search_ctx(SearchCtx& c) {
     dfree(c.vlog);
     dfree(c.qcounter);
     dfree(c.err);
-    c.v_slots = c.cg_entries = 0;
+    dfree(c.order);
+    c.v_slots = c.cg_entries = c.order_ints = 0;
 }
 
 // Scratch for a launch of `waves` waves with 2^vbits-slot visited tables.

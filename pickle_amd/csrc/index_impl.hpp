@@ -80,6 +80,8 @@ struct SearchCtx {
     int last_vbits = 0;  // visited-table size of the last search planned on this context
     int32_t* qcounter = nullptr;
     int32_t* err = nullptr;
+    int32_t* order = nullptr;  // fast mode's query order: keys, ranks, order, key counts
+    size_t order_ints = 0;
 };
 
 struct IoBuf {

@@ -764,14 +764,14 @@ static hipError_t fast_occupancy(int dtype, int rr, int block, size_t lds, int* 
 using namespace pk;
 
 namespace pk {
-int search_class_of(const pk_index* h);
 
-// The order for a batch of nq queries (order_queries in the comment above
-// kOrderMinQueries), or null when the batch is too small, no upper layer
-// gives enough queries per key, or the stream is being captured (the
-// buffers may need to grow).
-static int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, int size_class,
-                         const int32_t** out) {
+// The order for a batch of nq queries (see the comment above
+// kOrderMinQueries), enqueued on c's stream, or null when the batch is too
+// small, no upper layer gives enough queries per key, or the stream is
+// being captured (the buffers may need to grow).  Used by the fast and the
+// exact search; size_class is the fast mode's (search_class_of).
+int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, int size_class,
+                  const int32_t** out) {
     *out = nullptr;
     if (nq < kOrderMinQueries) return PK_OK;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

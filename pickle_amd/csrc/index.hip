@@ -674,6 +674,10 @@ int search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, cons
                          d_out_count, d_counters, &P, vbits_override);
     if (rc) return rc;
     if (nq == 0) return PK_OK;
+    // a large batch is handed out region by region (results do not depend
+    // on the order); graph captures and the static entry point keep the
+    // caller's order
+    if ((rc = order_queries(h, c, d_queries, nq, search_class_of(h), &P.a.order))) return rc;
     return enqueue_search(h, c, P);
 }
 }  // namespace pk

@@ -123,6 +123,10 @@ int fast_search_on(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq,
                    int32_t* d_out_ids, float* d_out_dists, int32_t* d_out_count, pk_query_counters* d_counters);
 // the context's work counter and failure flags, allocated once
 int ensure_ctx_counters(SearchCtx& c);
+// a processing order that groups a large batch's queries by region (fast.hip)
+int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, int size_class,
+                  const int32_t** out);
+int search_class_of(const pk_index* h);
 void free_search_ctx(SearchCtx& c);
 }  // namespace pk
 

@@ -53,8 +53,9 @@ __global__ __launch_bounds__(kSearchBlock, 4) void k_search(SearchArgs A) {
     for (;;) {
         int qi = 0;
         if (leader) qi = atomicAdd(A.qcounter, 1);
-        int64_t q = readlane(qi, 0);
-        if (q >= A.nq) break;
+        const int64_t pos = readlane(qi, 0);
+        if (pos >= A.nq) break;
+        const int64_t q = A.order ? (int64_t)uni(gbl(A.order)[pos]) : pos;
         const uint64_t t_begin = wall_clock64();
 
         // stage the query in LDS, zero padded to the row stride

@@ -48,6 +48,7 @@ struct SearchArgs {
     int32_t lds_per_wave;
     int32_t* qcounter;
     int32_t* err;
+    const int32_t* order;  // queue position -> query (null: in order)
 };
 
 int size_class_for(int dtype, int dim, int vstride);

@@ -98,12 +98,12 @@ def test_u8_m32_fast_mode_long_rows(u8_graph):
     idx.close()
 
 
-def test_fast_mode_query_order_does_not_change_results(u8_graph):
-    """A fast-mode batch of 8,192 queries or more is searched region by
-    region: each query is keyed by the upper-layer node its descent reaches
-    and the queue hands them out sorted by key.  The same queries searched
-    in batches of 4,096 (in the caller's order) return the same ids,
-    distances, counts and work counters, at every width."""
+def test_query_order_does_not_change_results(u8_graph):
+    """A batch of 8,192 queries or more is searched region by region: each
+    query is keyed by the upper-layer node its descent reaches and the queue
+    hands them out sorted by key.  The same queries searched in batches of
+    4,096 (in the caller's order) return the same ids, distances, counts and
+    work counters: the exact search, and the fast mode at every width."""
     base, queries, levels, g = u8_graph
     # 40,000 queries: at least 64 per node of the 607-node layer 1
     _, many = datasets.gaussian_mixture(base.shape[0], 40_000, base.shape[1], 64, 1, kind="u8")
@@ -111,6 +111,13 @@ def test_fast_mode_query_order_does_not_change_results(u8_graph):
     info = idx.info()
     say("u8 M32 layer sizes", info["sizes"], "queries", len(many))
     assert any(2 <= s <= len(many) // 64 for s in info["sizes"][1:]), "no upper layer to key on"
+    whole = idx.search(many, 10, 64, counters=True)
+    parts = [idx.search(many[s:s + 4096], 10, 64, counters=True) for s in range(0, len(many), 4096)]
+    say("u8 M32 query order, exact search")
+    for j in range(3):
+        np.testing.assert_array_equal(whole[j], np.concatenate([p[j] for p in parts]))
+    for f in ("hops", "neighbors", "dists"):
+        np.testing.assert_array_equal(whole[3][f], np.concatenate([p[3][f] for p in parts]))
     for width in (1, 2, 4):
         idx.set_fast_tuning(width=width)
         whole = idx.search_fast(many, 10, 128, counters=True)

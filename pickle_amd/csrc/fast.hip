@@ -657,10 +657,8 @@ static hipError_t launch_fast(const FastArgs& a, int dtype, int metric, int size
 // queries are therefore handed out region by region: each query descends
 // the upper layers as the search does (greedy_fast) down to a key layer
 // whose node count gives at least kOrderQueriesPerKey queries per node on
-// average.  Its key is the node it stops at there, after the node it
-// stopped at one layer up (coarse region first, so that the fine regions
-// running together are neighbours), and a counting sort of the keys gives
-// the order.  Results do not depend on the order.
+// average, the node it stops at is its key, and a counting sort of the keys
+// gives the order.  Results do not depend on the order.
 constexpr int64_t kOrderMinQueries = 8192;
 constexpr int64_t kOrderQueriesPerKey = 64;
 
@@ -668,8 +666,7 @@ struct OrderArgs {
     const LayerDev* layers;
     int32_t enter_level;
     int32_t key_level;
-    int32_t nfine;    // nodes of the key layer
-    int32_t ncoarse;  // nodes of the layer above it (1: no coarse key)
+    int32_t nkeys;
     int32_t dim;
     int32_t vstride;
     int32_t qbytes;
@@ -696,15 +693,14 @@ __global__ __launch_bounds__(256) void k_order_keys(OrderArgs A) {
         DF df;
         df.load_query(qst, g, IS_INT ? (A.vstride >> 4) : A.dim);
         WorkCount wc;
-        int cur = 0, coarse = 0;
+        int cur = 0;
         for (int level = A.enter_level; level > A.key_level; level--) {
             const LayerDev L = A.layers[level];
             cur = greedy_fast(df, L, A.vstride, cur, cl, cd, lane, wc);
-            if (level == A.key_level + 1) coarse = cur < A.ncoarse ? cur : A.ncoarse - 1;
             cur = gbl(L.down)[cur];
         }
         cur = greedy_fast(df, A.layers[A.key_level], A.vstride, cur, cl, cd, lane, wc);
-        const int key = coarse * A.nfine + (cur < A.nfine ? cur : A.nfine - 1);
+        const int key = cur < A.nkeys ? cur : A.nkeys - 1;
         if (lane == 0) {
             A.keys[q] = key;
             A.ranks[q] = atomicAdd(&A.hist[key], 1);
@@ -790,10 +786,7 @@ int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, 
         }
     }
     if (lk == 0) return PK_OK;
-    const int64_t nfine = h->layers[lk].n;
-    int64_t ncoarse = lk < enter && lk + 1 < PK_MAX_LEVELS ? std::max<int64_t>(h->layers[lk + 1].n, 1) : 1;
-    if (nfine * ncoarse > ((int64_t)1 << 22)) ncoarse = 1;
-    const int64_t nkeys = nfine * ncoarse;
+    const int64_t nkeys = h->layers[lk].n;
     const size_t need = (size_t)3 * (size_t)nq + (size_t)nkeys;
     if (c.order_ints < need) {
         if (c.order) {
@@ -817,8 +810,7 @@ int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, 
     a.layers = static_cast<const LayerDev*>(h->d_layers);
     a.enter_level = enter;
     a.key_level = lk;
-    a.nfine = (int32_t)nfine;
-    a.ncoarse = (int32_t)ncoarse;
+    a.nkeys = (int32_t)nkeys;
     a.dim = h->dim;
     a.vstride = h->vstride;
     a.qbytes = h->qbytes;

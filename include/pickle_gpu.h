@@ -136,8 +136,8 @@ int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, i
  * returns PK_ERR_DEVICE only if that still fails or captured search graphs
  * keep the scratch from growing.  The pk_search_device* calls report such
  * queries through their counts and pk_device_error instead.
- * Batches of 8,192 queries or more (here and in pk_search_device*,
- * pk_search_fast*) are processed region by region: each query is keyed by
+ * Batches of 8,192 queries or more (here, in pk_search_device and
+ * pk_search_device_ex, and in pk_search_fast*) are processed region by region: each query is keyed by
  * the upper-layer node its descent reaches, and the keys are sorted on the
  * device, so that queries sharing rows run together.  Results, counts and
  * counters are per query and do not depend on the order.  The first such

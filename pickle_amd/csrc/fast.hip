@@ -654,19 +654,19 @@ static hipError_t launch_fast(const FastArgs& a, int dtype, int metric, int size
 // Queries of one region of the data visit overlapping rows, and rows read by
 // one query are still in the L2 or the MI355X's 256 MB Infinity Cache when
 // another query of the region asks for them.  For a large batch the
-// queries are therefore handed out region by region: each query descends
-// the upper layers as the search does (greedy_fast) down to a key layer
-// whose node count gives at least kOrderQueriesPerKey queries per node on
-// average, the node it stops at is its key, and a counting sort of the keys
-// gives the order.  Results do not depend on the order.
+// queries are therefore handed out region by region: the pivots are the
+// first nodes of an upper layer (a random sample of the rows, as level
+// sampling is), about one per kOrderQueriesPerKey queries and at most
+// kOrderMaxKeys; a query's key is its nearest pivot, and a counting sort of
+// the keys gives the order.  Results do not depend on the order.
 constexpr int64_t kOrderMinQueries = 8192;
-constexpr int64_t kOrderQueriesPerKey = 64;
+constexpr int64_t kOrderQueriesPerKey = 1536;
+constexpr int64_t kOrderMaxKeys = 64;
 
 struct OrderArgs {
     const LayerDev* layers;
-    int32_t enter_level;
-    int32_t key_level;
-    int32_t nkeys;
+    int32_t piv_level;  // the pivots are nodes [0, npiv) of this layer
+    int32_t npiv;
     int32_t dim;
     int32_t vstride;
     int32_t qbytes;
@@ -692,15 +692,14 @@ __global__ __launch_bounds__(256) void k_order_keys(OrderArgs A) {
         stage_row(qst, A.queries + (size_t)q * (size_t)A.qbytes, A.qbytes, A.vstride, lane);
         DF df;
         df.load_query(qst, g, IS_INT ? (A.vstride >> 4) : A.dim);
-        WorkCount wc;
-        int cur = 0;
-        for (int level = A.enter_level; level > A.key_level; level--) {
-            const LayerDev L = A.layers[level];
-            cur = greedy_fast(df, L, A.vstride, cur, cl, cd, lane, wc);
-            cur = gbl(L.down)[cur];
-        }
-        cur = greedy_fast(df, A.layers[A.key_level], A.vstride, cur, cl, cd, lane, wc);
-        const int key = cur < A.nkeys ? cur : A.nkeys - 1;
+        // the nearest pivot (npiv <= 64: one batch of distances)
+        if (lane < A.npiv) cl[lane] = lane;
+        __builtin_amdgcn_wave_barrier();
+        batch_distances(df, A.layers[A.piv_level].vec, A.vstride, cl, cd, A.npiv, lane);
+        __builtin_amdgcn_wave_barrier();
+        float m;
+        const int at = wave_min_lane(lane < A.npiv ? cd[lane] : __builtin_inff(), lane < A.npiv, &m);
+        const int key = at >= 0 ? at : 0;
         if (lane == 0) {
             A.keys[q] = key;
             A.ranks[q] = atomicAdd(&A.hist[key], 1);
@@ -776,17 +775,16 @@ int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, 
     if (nq < kOrderMinQueries) return PK_OK;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(c.stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return PK_OK;
+    const int64_t nkeys = std::min<int64_t>(kOrderMaxKeys, std::max<int64_t>(2, nq / kOrderQueriesPerKey));
+    // the highest upper layer with that many nodes
     const int enter = h->enter_level;
-    int lk = 0;
-    for (int l = 1; l <= enter && l < PK_MAX_LEVELS; l++) {
-        const int64_t n = h->layers[l].n;
-        if (n >= 2 && n <= nq / kOrderQueriesPerKey) {
-            lk = l;
+    int lp = 0;
+    for (int l = std::min(enter, PK_MAX_LEVELS - 1); l >= 1; l--)
+        if (h->layers[l].n >= nkeys) {
+            lp = l;
             break;
         }
-    }
-    if (lk == 0) return PK_OK;
-    const int64_t nkeys = h->layers[lk].n;
+    if (lp == 0) return PK_OK;
     const size_t need = (size_t)3 * (size_t)nq + (size_t)nkeys;
     if (c.order_ints < need) {
         if (c.order) {
@@ -808,9 +806,8 @@ int order_queries(pk_index* h, SearchCtx& c, const void* d_queries, int64_t nq, 
     int32_t* hist = order + nq;
     OrderArgs a{};
     a.layers = static_cast<const LayerDev*>(h->d_layers);
-    a.enter_level = enter;
-    a.key_level = lk;
-    a.nkeys = (int32_t)nkeys;
+    a.piv_level = lp;
+    a.npiv = (int32_t)nkeys;
     a.dim = h->dim;
     a.vstride = h->vstride;
     a.qbytes = h->qbytes;

@@ -137,12 +137,13 @@ int pk_index_read_layer(pk_index *h, int level, int32_t *ext, int32_t *degree, i
  * keep the scratch from growing.  The pk_search_device* calls report such
  * queries through their counts and pk_device_error instead.
  * Batches of 8,192 queries or more (here, in pk_search_device and
- * pk_search_device_ex, and in pk_search_fast*) are processed region by region: each query is keyed by
- * the upper-layer node its descent reaches, and the keys are sorted on the
- * device, so that queries sharing rows run together.  Results, counts and
- * counters are per query and do not depend on the order.  The first such
- * call on a handle allocates 12 bytes per query for it (kept and grown with
- * the handle); calls during stream capture keep the caller's order. */
+ * pk_search_device_ex, and in pk_search_fast*) are processed region by
+ * region: each query is keyed by its nearest of up to 64 pivots (the first
+ * nodes of an upper layer), and the keys are sorted on the device, so that
+ * queries sharing rows run together.  Results, counts and counters are per
+ * query and do not depend on the order.  The first such call on a handle
+ * allocates about 12 bytes per query for it (kept and grown with the
+ * handle); calls during stream capture keep the caller's order. */
 int pk_search(pk_index *h, const void *queries, int64_t nq, const int32_t *k, const int32_t *ef,
               int32_t k_all, int32_t ef_all, int32_t k_stride, int32_t *out_ids, float *out_dists,
               int32_t *out_count, pk_query_counters *counters);

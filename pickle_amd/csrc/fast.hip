@@ -662,6 +662,9 @@ static hipError_t launch_fast(const FastArgs& a, int dtype, int metric, int size
 constexpr int64_t kOrderMinQueries = 8192;
 constexpr int64_t kOrderQueriesPerKey = 1536;
 constexpr int64_t kOrderMaxKeys = 64;
+// k_order_keys holds the pivot ids and distances in one wave-wide batch
+// (cl / cd, kWave entries each): more pivots need a loop over batches
+static_assert(kOrderMaxKeys <= kWave, "the pivot scan takes at most one wave-wide batch");
 
 struct OrderArgs {
     const LayerDev* layers;
